@@ -1,0 +1,153 @@
+// ct_internal.hpp - shared device/host helpers of the MI355X CT core.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "../../include/ct_core.h"
+
+namespace ct {
+
+// ---------------------------------------------------------------- storage
+template <typename S> struct St;
+template <> struct St<float> {
+    __device__ __forceinline__ static float ld(const float *p, int64_t i) { return p[i]; }
+    __device__ __forceinline__ static void st(float *p, int64_t i, float v) { p[i] = v; }
+};
+template <> struct St<double> {
+    __device__ __forceinline__ static double ld(const double *p, int64_t i) { return p[i]; }
+    __device__ __forceinline__ static void st(double *p, int64_t i, double v) { p[i] = v; }
+};
+template <> struct St<__half> {
+    __device__ __forceinline__ static float ld(const __half *p, int64_t i) { return __half2float(p[i]); }
+    __device__ __forceinline__ static void st(__half *p, int64_t i, float v) { p[i] = __float2half_rn(v); }
+};
+template <> struct St<__hip_bfloat16> {
+    __device__ __forceinline__ static float ld(const __hip_bfloat16 *p, int64_t i) { return __bfloat162float(p[i]); }
+    __device__ __forceinline__ static void st(__hip_bfloat16 *p, int64_t i, float v) { p[i] = __float2bfloat16(v); }
+};
+
+template <typename S> __device__ __forceinline__ float ldf(const S *p, int64_t i) { return St<S>::ld(p, i); }
+
+inline size_t dtype_size(ct_dtype d) {
+    switch (d) {
+    case CT_F32: return 4;
+    case CT_F16: return 2;
+    case CT_BF16: return 2;
+    case CT_F64: return 8;
+    }
+    return 0;
+}
+inline bool dtype_valid(int d) { return d == CT_F32 || d == CT_F16 || d == CT_BF16 || d == CT_F64; }
+
+// ------------------------------------------------------------ geometry
+// Derived constants, computed on the host exactly as V63/main_recon.cu:61-111
+// computes its compile-time constants, in the arithmetic type T.
+template <typename T> struct DGeo {
+    int M, N, Z, R, ZP, NS, moff, tb;
+    T vx, vy, vz, ivx, ivy, ivz, bx, by, bz;
+    T du, dv, idu, idv, ymin, zmin, zmax, dso, dod, sx, sy, sz0, sdz;
+    T us_rate, ang0, shift;
+};
+
+template <typename T> DGeo<T> derive(const ct_geometry &g) {
+    DGeo<T> d;
+    d.M = g.nx; d.N = g.ny; d.Z = g.nz; d.R = g.ncols; d.ZP = g.nrows;
+    d.NS = g.nsources < 1 ? 1 : g.nsources; d.moff = g.x_offset; d.tb = d.NS > 1;
+    d.vx = (T)g.vx; d.vy = (T)g.vy; d.vz = (T)g.vz;
+    d.ivx = (T)(1.0 / (double)d.vx); d.ivy = (T)(1.0 / (double)d.vy); d.ivz = (T)(1.0 / (double)d.vz);
+    d.bx = -d.vx * ((T)d.M / (T)2 + (T)d.moff);
+    d.by = -d.vy * ((T)d.N / (T)2);
+    d.bz = -d.vz * ((T)d.Z / (T)2);
+    d.du = (T)g.du; d.dv = (T)g.dv;
+    d.idu = (T)1 / d.du; d.idv = (T)1 / d.dv;
+    const T off = (T)g.det_offset;
+    d.ymin = -d.du * ((T)d.R / (T)2 - (T)0.5) + off;
+    d.zmin = -d.dv * ((T)d.ZP / (T)2 - (T)0.5);
+    d.zmax = +d.dv * ((T)d.ZP / (T)2 - (T)0.5);
+    d.dso = (T)g.dso; d.dod = (T)g.dod;
+    d.sx = d.dso; d.sy = off;
+    d.sz0 = (T)g.src_z0; d.sdz = (T)g.src_dz;
+    d.us_rate = (T)g.angle_step_deg; d.ang0 = (T)g.angle0_deg; d.shift = (T)g.angle_shift_deg;
+    return d;
+}
+
+// --------------------------------------------------------------- context
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace ct
+
+struct ct_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // grow-only scratch slots, named by use
+    ct::DevBuf slots[16];
+    // cached per-view angle table (sin, cos interleaved) keyed by geometry + precision + fdk
+    ct::DevBuf angles;
+    std::vector<double> angle_key;
+    size_t scratch_total = 0;
+};
+
+namespace ct {
+
+enum Slot {
+    SLOT_CORR = 0,     // SART correction images
+    SLOT_RED = 1,      // reduction partials
+    SLOT_FGP = 2,      // 7 FGP volumes
+    SLOT_FDK = 3,      // weighted + filtered projection stack
+    SLOT_TMP_VOL = 4,  // adjoint test / power iteration volumes
+    SLOT_TMP_VOL2 = 5,
+    SLOT_TMP_PROJ = 6,
+    SLOT_TMP_PROJ2 = 7,
+    SLOT_FFT = 8,      // twiddles
+    SLOT_MISC = 9
+};
+
+ct_status fail(ct_context *ctx, ct_status s, const std::string &msg);
+ct_status hip_check(ct_context *ctx, hipError_t e, const char *what);
+ct_status launch_check(ct_context *ctx, const char *what);
+void *slot(ct_context *ctx, int which, size_t bytes, ct_status *st);
+// Device angle table (T = float or double), sin at [2v], cos at [2v+1].
+const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fdk, ct_status *st);
+ct_status check_ptr(ct_context *ctx, const void *p, ct_dtype dt, const char *name);
+
+// launchers implemented per translation unit
+ct_status launch_forward(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt,
+                         void *out, ct_dtype odt, const void *meas, ct_dtype mdt,
+                         int32_t view_first, int32_t view_stride, int32_t nviews, int cmd);
+ct_status launch_voxel_bp(ct_context *ctx, const ct_geometry &g, void *vol, ct_dtype vdt,
+                          const void *corr, int32_t view_first, int32_t view_stride,
+                          int32_t nviews, double beta, int mode);
+ct_status launch_adjoint(ct_context *ctx, const ct_geometry &g, const void *proj, ct_dtype pdt,
+                         void *vol, ct_dtype vdt, int32_t view0, int32_t nviews);
+ct_status launch_fill(ct_context *ctx, void *p, ct_dtype dt, int64_t n, double value);
+ct_status launch_axpby(ct_context *ctx, const void *x, void *y, ct_dtype dt, int64_t n, double a, double b);
+ct_status launch_randn(ct_context *ctx, void *p, ct_dtype dt, int64_t n, uint64_t seed);
+ct_status reduce_dot(ct_context *ctx, const void *a, const void *b, int64_t n, ct_dtype dt, int mode, double *out);
+ct_status reduce_tv(ct_context *ctx, const void *f, ct_dtype dt, int M, int N, int Z, int mode, double eps, double *out);
+ct_status launch_convert(ct_context *ctx, const void *in, ct_dtype idt, void *out, ct_dtype odt, int64_t n);
+ct_status launch_forward_mv(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
+                            ct_dtype odt, const void *meas, ct_dtype mdt, int32_t first, int32_t stride,
+                            int32_t nv, int cmd, int32_t mv0);
+ct_status launch_tv_eps_grad(ct_context *ctx, const void *f, void *g, ct_dtype dt, int M, int N, int Z, double eps);
+ct_status launch_fista(ct_context *ctx, void *x, void *xp, void *y, int64_t n, ct_dtype dt, double gamma);
+ct_status launch_fgp(ct_context *ctx, int M, int N, int Z, void *vol, ct_dtype dt, double lambda, int niter);
+ct_status launch_fgp_D(ct_context *ctx, int M, int N, int Z, const void *x, void *r, void *s, void *t, ct_dtype dt);
+ct_status launch_fgp_Dt(ct_context *ctx, int M, int N, int Z, const void *r, const void *s, const void *t, void *out, ct_dtype dt);
+ct_status launch_fdk_weight(ct_context *ctx, const ct_geometry &g, const void *in, ct_dtype idt, void *out, ct_dtype odt);
+ct_status launch_ramp(ct_context *ctx, const ct_geometry &g, const void *in, ct_dtype idt, void *out, ct_dtype odt,
+                      int32_t nimg, ct_window window);
+ct_status launch_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, ct_dtype pdt, void *vol, ct_dtype vdt,
+                        int32_t view0, int32_t nviews);
+
+} // namespace ct

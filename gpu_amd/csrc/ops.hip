@@ -1,0 +1,431 @@
+// ops.hip - fused elementwise, stencil and FP64 reduction kernels:
+// FISTA momentum, TV-FGP, the FGP difference pair D / D^T, TV_eps and its
+// gradient, TV / residual / dot norms, fills, conversions and seeded normals.
+#include "ct_internal.hpp"
+#include <cmath>
+#include <type_traits>
+
+namespace ct {
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+// --------------------------------------------------------------- basics
+template <typename S>
+__global__ void k_fill(S *p, int64_t n, float v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        St<S>::st(p, i, v);
+}
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static dim3 grid_for(int64_t n, int threads = 256) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+ct_status launch_fill(ct_context *ctx, void *p, ct_dtype dt, int64_t n, double value) {
+    if (n <= 0) return CT_OK;
+    dim3 g = grid_for(n), b(256);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_fill<float>, g, b, 0, ctx->stream, (float *)p, n, (float)value); break;
+    case CT_F16: hipLaunchKernelGGL(k_fill<__half>, g, b, 0, ctx->stream, (__half *)p, n, (float)value); break;
+    case CT_BF16: hipLaunchKernelGGL(k_fill<__hip_bfloat16>, g, b, 0, ctx->stream, (__hip_bfloat16 *)p, n, (float)value); break;
+    case CT_F64: hipLaunchKernelGGL(k_fill_f64, g, b, 0, ctx->stream, (double *)p, n, value); break;
+    }
+    return launch_check(ctx, "fill");
+}
+
+template <typename SI, typename SO>
+__global__ void k_convert(const SI *in, SO *out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if constexpr (std::is_same<SI, double>::value && std::is_same<SO, double>::value) out[i] = in[i];
+        else if constexpr (std::is_same<SI, double>::value) St<SO>::st(out, i, (float)in[i]);
+        else if constexpr (std::is_same<SO, double>::value) out[i] = (double)ldf(in, i);
+        else St<SO>::st(out, i, ldf(in, i));
+    }
+}
+
+template <typename SI>
+static ct_status conv_out(ct_context *ctx, const void *in, void *out, ct_dtype odt, int64_t n) {
+    dim3 g = grid_for(n), b(256);
+    switch (odt) {
+    case CT_F32: hipLaunchKernelGGL((k_convert<SI, float>), g, b, 0, ctx->stream, (const SI *)in, (float *)out, n); break;
+    case CT_F16: hipLaunchKernelGGL((k_convert<SI, __half>), g, b, 0, ctx->stream, (const SI *)in, (__half *)out, n); break;
+    case CT_BF16: hipLaunchKernelGGL((k_convert<SI, __hip_bfloat16>), g, b, 0, ctx->stream, (const SI *)in, (__hip_bfloat16 *)out, n); break;
+    case CT_F64: hipLaunchKernelGGL((k_convert<SI, double>), g, b, 0, ctx->stream, (const SI *)in, (double *)out, n); break;
+    }
+    return launch_check(ctx, "convert");
+}
+
+ct_status launch_convert(ct_context *ctx, const void *in, ct_dtype idt, void *out, ct_dtype odt, int64_t n) {
+    if (n <= 0) return CT_OK;
+    switch (idt) {
+    case CT_F32: return conv_out<float>(ctx, in, out, odt, n);
+    case CT_F16: return conv_out<__half>(ctx, in, out, odt, n);
+    case CT_BF16: return conv_out<__hip_bfloat16>(ctx, in, out, odt, n);
+    case CT_F64: return conv_out<double>(ctx, in, out, odt, n);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "convert: bad dtype");
+}
+
+// y = a*x + b*y
+template <typename S>
+__global__ void k_axpby(const S *x, S *y, int64_t n, float a, float b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        St<S>::st(y, i, a * ldf(x, i) + b * ldf(y, i));
+}
+__global__ void k_axpby_f64(const double *x, double *y, int64_t n, double a, double b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+ct_status launch_axpby(ct_context *ctx, const void *x, void *y, ct_dtype dt, int64_t n, double a, double b) {
+    dim3 g = grid_for(n), bl(256);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_axpby<float>, g, bl, 0, ctx->stream, (const float *)x, (float *)y, n, (float)a, (float)b); break;
+    case CT_F16: hipLaunchKernelGGL(k_axpby<__half>, g, bl, 0, ctx->stream, (const __half *)x, (__half *)y, n, (float)a, (float)b); break;
+    case CT_BF16: hipLaunchKernelGGL(k_axpby<__hip_bfloat16>, g, bl, 0, ctx->stream, (const __hip_bfloat16 *)x, (__hip_bfloat16 *)y, n, (float)a, (float)b); break;
+    case CT_F64: hipLaunchKernelGGL(k_axpby_f64, g, bl, 0, ctx->stream, (const double *)x, (double *)y, n, a, b); break;
+    }
+    return launch_check(ctx, "axpby");
+}
+
+// Seeded N(0,1): splitmix64 of (seed, index) -> two uniforms -> Box-Muller.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+template <typename S>
+__global__ void k_randn(S *p, int64_t n, uint64_t seed) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t a = splitmix64(seed ^ splitmix64((uint64_t)i * 2 + 1));
+        uint64_t b = splitmix64(a + 0x632BE59BD9B4E019ull);
+        double u1 = ((double)(a >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        double u2 = ((double)(b >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        double r = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        if constexpr (std::is_same<S, double>::value) p[i] = r;
+        else St<S>::st(p, i, (float)r);
+    }
+}
+ct_status launch_randn(ct_context *ctx, void *p, ct_dtype dt, int64_t n, uint64_t seed) {
+    dim3 g = grid_for(n), b(256);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_randn<float>, g, b, 0, ctx->stream, (float *)p, n, seed); break;
+    case CT_F16: hipLaunchKernelGGL(k_randn<__half>, g, b, 0, ctx->stream, (__half *)p, n, seed); break;
+    case CT_BF16: hipLaunchKernelGGL(k_randn<__hip_bfloat16>, g, b, 0, ctx->stream, (__hip_bfloat16 *)p, n, seed); break;
+    case CT_F64: hipLaunchKernelGGL(k_randn<double>, g, b, 0, ctx->stream, (double *)p, n, seed); break;
+    }
+    return launch_check(ctx, "randn");
+}
+
+// ------------------------------------------------------------ reductions
+// Per-block FP64 partial sums (wave64 shuffles, then LDS across the 4 waves),
+// then one block folds the partials in fixed order: deterministic.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[RED_THREADS / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) part[wid] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < RED_THREADS / 64; ++w) r += part[w];
+    return r;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_fold(const double *part, int n, double *out) {
+    double v = 0;
+    for (int i = threadIdx.x; i < n; i += RED_THREADS) v += part[i];
+    double r = block_sum(v);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+// mode 0: sum a*b; mode 1: sum (a-b)^2
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void k_dot(const S *a, const S *b, int64_t n, int mode, double *part) {
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+        double x, y;
+        if constexpr (std::is_same<S, double>::value) { x = a[i]; y = b[i]; }
+        else { x = (double)ldf(a, i); y = (double)ldf(b, i); }
+        if (mode == 0) acc += x * y;
+        else { double e = x - y; acc += e * e; }
+    }
+    double r = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+static ct_status finish_reduce(ct_context *ctx, double *part, int nblk, double *out) {
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(RED_THREADS), 0, ctx->stream, part, nblk, part + RED_BLOCKS);
+    ct_status st = launch_check(ctx, "reduce");
+    if (st != CT_OK) return st;
+    st = hip_check(ctx, hipMemcpyAsync(out, part + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "reduce copy");
+    if (st != CT_OK) return st;
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "reduce sync");
+}
+
+static int red_blocks(int64_t n) {
+    int64_t b = (n + RED_THREADS - 1) / RED_THREADS;
+    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    return b < 1 ? 1 : (int)b;
+}
+
+ct_status reduce_dot(ct_context *ctx, const void *a, const void *b, int64_t n, ct_dtype dt, int mode, double *out) {
+    ct_status st;
+    double *part = (double *)slot(ctx, SLOT_RED, sizeof(double) * (RED_BLOCKS + 8), &st);
+    if (!part) return st;
+    const int nb = red_blocks(n);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const float *)a, (const float *)b, n, mode, part); break;
+    case CT_F16: hipLaunchKernelGGL(k_dot<__half>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const __half *)a, (const __half *)b, n, mode, part); break;
+    case CT_BF16: hipLaunchKernelGGL(k_dot<__hip_bfloat16>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const __hip_bfloat16 *)a, (const __hip_bfloat16 *)b, n, mode, part); break;
+    case CT_F64: hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const double *)a, (const double *)b, n, mode, part); break;
+    }
+    st = launch_check(ctx, "dot");
+    if (st != CT_OK) return st;
+    return finish_reduce(ctx, part, nb, out);
+}
+
+template <typename S> __device__ __forceinline__ double ld64(const S *p, int64_t i) {
+    if constexpr (std::is_same<S, double>::value) return p[i];
+    else return (double)ldf(p, i);
+}
+
+// mode 0: isotropic forward-difference TV (V63/kernel_tool_functions.cu:39-65);
+// mode 1: TV_eps with the backward differences that exist.
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void k_tv(const S *f, int M, int N, int Z, int mode, double eps, double *part) {
+    const int64_t MN = (int64_t)M * N, V = MN * Z;
+    double acc = 0;
+    for (int64_t v = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; v < V; v += (int64_t)gridDim.x * RED_THREADS) {
+        const int i = (int)(v % M), j = (int)((v / M) % N), k = (int)(v / MN);
+        const double c = ld64(f, v);
+        if (mode == 0) {
+            double dx = i < M - 1 ? c - ld64(f, v + 1) : 0.0;
+            double dy = j < N - 1 ? c - ld64(f, v + M) : 0.0;
+            double dz = k < Z - 1 ? c - ld64(f, v + MN) : 0.0;
+            acc += sqrt(dx * dx + dy * dy + dz * dz);
+        } else {
+            double q = eps;
+            if (i > 0) { double e = c - ld64(f, v - 1); q += e * e; }
+            if (j > 0) { double e = c - ld64(f, v - M); q += e * e; }
+            if (k > 0) { double e = c - ld64(f, v - MN); q += e * e; }
+            acc += sqrt(q);
+        }
+    }
+    double r = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+ct_status reduce_tv(ct_context *ctx, const void *f, ct_dtype dt, int M, int N, int Z, int mode, double eps, double *out) {
+    ct_status st;
+    double *part = (double *)slot(ctx, SLOT_RED, sizeof(double) * (RED_BLOCKS + 8), &st);
+    if (!part) return st;
+    const int nb = red_blocks((int64_t)M * N * Z);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_tv<float>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const float *)f, M, N, Z, mode, eps, part); break;
+    case CT_F16: hipLaunchKernelGGL(k_tv<__half>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const __half *)f, M, N, Z, mode, eps, part); break;
+    case CT_BF16: hipLaunchKernelGGL(k_tv<__hip_bfloat16>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const __hip_bfloat16 *)f, M, N, Z, mode, eps, part); break;
+    case CT_F64: hipLaunchKernelGGL(k_tv<double>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, (const double *)f, M, N, Z, mode, eps, part); break;
+    }
+    st = launch_check(ctx, "tv");
+    if (st != CT_OK) return st;
+    return finish_reduce(ctx, part, nb, out);
+}
+
+// d TV_eps / d x_v: own term minus the terms of the three forward neighbours.
+template <typename S>
+__global__ void k_tv_eps_grad(const S *f, S *g, int M, int N, int Z, double eps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+    auto phi = [&](int a, int b, int cc, int64_t w) {
+        const double c = ld64(f, w);
+        double q = eps;
+        if (a > 0) { double e = c - ld64(f, w - 1); q += e * e; }
+        if (b > 0) { double e = c - ld64(f, w - M); q += e * e; }
+        if (cc > 0) { double e = c - ld64(f, w - MN); q += e * e; }
+        return sqrt(q);
+    };
+    const double c = ld64(f, v);
+    double num = 0;
+    if (i > 0) num += c - ld64(f, v - 1);
+    if (j > 0) num += c - ld64(f, v - M);
+    if (k > 0) num += c - ld64(f, v - MN);
+    double gv = num / phi(i, j, k, v);
+    if (i < M - 1) gv -= (ld64(f, v + 1) - c) / phi(i + 1, j, k, v + 1);
+    if (j < N - 1) gv -= (ld64(f, v + M) - c) / phi(i, j + 1, k, v + M);
+    if (k < Z - 1) gv -= (ld64(f, v + MN) - c) / phi(i, j, k + 1, v + MN);
+    if constexpr (std::is_same<S, double>::value) g[v] = gv;
+    else St<S>::st(g, v, (float)gv);
+}
+
+ct_status launch_tv_eps_grad(ct_context *ctx, const void *f, void *g, ct_dtype dt, int M, int N, int Z, double eps) {
+    dim3 b(256), gr((M + 255) / 256, N, Z);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_tv_eps_grad<float>, gr, b, 0, ctx->stream, (const float *)f, (float *)g, M, N, Z, eps); break;
+    case CT_F16: hipLaunchKernelGGL(k_tv_eps_grad<__half>, gr, b, 0, ctx->stream, (const __half *)f, (__half *)g, M, N, Z, eps); break;
+    case CT_BF16: hipLaunchKernelGGL(k_tv_eps_grad<__hip_bfloat16>, gr, b, 0, ctx->stream, (const __hip_bfloat16 *)f, (__hip_bfloat16 *)g, M, N, Z, eps); break;
+    case CT_F64: hipLaunchKernelGGL(k_tv_eps_grad<double>, gr, b, 0, ctx->stream, (const double *)f, (double *)g, M, N, Z, eps); break;
+    }
+    return launch_check(ctx, "tv_eps_grad");
+}
+
+// ------------------------------------------------------------ FISTA
+// y = x + gamma*(x - x_prev); x_prev = x (V63/main_recon.cu:375-382), one pass.
+template <typename S, typename T>
+__global__ void k_fista(S *x, S *xp, S *y, int64_t n, T gamma) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T xi, pi;
+        if constexpr (std::is_same<S, double>::value) { xi = x[i]; pi = xp[i]; }
+        else { xi = ldf(x, i); pi = ldf(xp, i); }
+        const T yi = xi + gamma * (xi - pi);
+        if constexpr (std::is_same<S, double>::value) { xp[i] = xi; y[i] = yi; }
+        else { St<S>::st(xp, i, xi); St<S>::st(y, i, yi); }
+    }
+}
+ct_status launch_fista(ct_context *ctx, void *x, void *xp, void *y, int64_t n, ct_dtype dt, double gamma) {
+    dim3 g = grid_for(n), b(256);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL((k_fista<float, float>), g, b, 0, ctx->stream, (float *)x, (float *)xp, (float *)y, n, (float)gamma); break;
+    case CT_F16: hipLaunchKernelGGL((k_fista<__half, float>), g, b, 0, ctx->stream, (__half *)x, (__half *)xp, (__half *)y, n, (float)gamma); break;
+    case CT_BF16: hipLaunchKernelGGL((k_fista<__hip_bfloat16, float>), g, b, 0, ctx->stream, (__hip_bfloat16 *)x, (__hip_bfloat16 *)xp, (__hip_bfloat16 *)y, n, (float)gamma); break;
+    case CT_F64: hipLaunchKernelGGL((k_fista<double, double>), g, b, 0, ctx->stream, (double *)x, (double *)xp, (double *)y, n, gamma); break;
+    }
+    return launch_check(ctx, "fista");
+}
+
+// ------------------------------------------------------------ TV-FGP
+// Dual fields r,s,t (and the momentum copies o,p,q) are volume-sized arrays in
+// the arithmetic type: r[v] is the x-face between v and v+e_x, zero on the last
+// x plane (the reference's staggered d_r[i+1], V63/host_FGP_Denoise_GPUx7.c:8).
+// L = b - lambda * D^T(r,s,t)   (FGP_operator_L_Pc, V63/kernel_FGP_Denoise_GPUx7.cu:2-24)
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_fgp_L(const S *b, const T *r, const T *s, const T *t, S *Lout, T *Lt,
+                                               int M, int N, int Z, T lambda) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+    const T rm = i > 0 ? r[v - 1] : (T)0;
+    const T sm = j > 0 ? s[v - M] : (T)0;
+    const T tm = k > 0 ? t[v - MN] : (T)0;
+    const T tl = (r[v] - rm) + (s[v] - sm) + (t[v] - tm);
+    T bv;
+    if constexpr (std::is_same<S, double>::value) bv = b[v]; else bv = ldf(b, v);
+    const T L = bv - lambda * tl;
+    if (Lt) Lt[v] = L;
+    if (Lout) { if constexpr (std::is_same<S, double>::value) Lout[v] = L; else St<S>::st(Lout, v, L); }
+}
+
+// dual gradient step, unit-ball projection and momentum
+// (FGP_operator_LT_Pp, V63/kernel_FGP_Denoise_GPUx7.cu:27-88)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fgp_LT(T *r, T *s, T *t, const T *L, T *o, T *p, T *q, int M, int N, int Z,
+                                                T g, T a12) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+    T tr = r[v], ts = s[v], tt = t[v];
+    const T Lv = L[v];
+    if (i < M - 1) tr = tr + a12 * (Lv - L[v + 1]);
+    if (j < N - 1) ts = ts + a12 * (Lv - L[v + M]);
+    if (k < Z - 1) tt = tt + a12 * (Lv - L[v + MN]);
+    const T nrm = sqrt(tr * tr + ts * ts + tt * tt);
+    const T den = nrm > (T)1 ? nrm : (T)1;
+    if (i < M - 1) { const T on = tr / den; r[v] = on + g * (on - o[v]); o[v] = on; }
+    if (j < N - 1) { const T pn = ts / den; s[v] = pn + g * (pn - p[v]); p[v] = pn; }
+    if (k < Z - 1) { const T qn = tt / den; t[v] = qn + g * (qn - q[v]); q[v] = qn; }
+}
+
+template <typename S, typename T>
+static ct_status run_fgp(ct_context *ctx, int M, int N, int Z, void *vol, double lambda, int niter) {
+    const int64_t V = (int64_t)M * N * Z;
+    ct_status st;
+    T *w = (T *)slot(ctx, SLOT_FGP, sizeof(T) * V * 7, &st);
+    if (!w) return st;
+    T *r = w, *s = w + V, *t = w + 2 * V, *o = w + 3 * V, *p = w + 4 * V, *q = w + 5 * V, *L = w + 6 * V;
+    st = launch_fill(ctx, w, sizeof(T) == 8 ? CT_F64 : CT_F32, V * 6, 0.0);
+    if (st != CT_OK) return st;
+    dim3 b(256), gr((M + 255) / 256, N, Z);
+    const T lam = (T)lambda;
+    const T a12 = (T)1 / (lam * (T)12);
+    T tk1 = 1;
+    for (int it = 0; it < niter; ++it) {
+        const T tk = ((T)1 + std::sqrt((T)1 + (T)4 * tk1 * tk1)) / (T)2;
+        const T g = (tk1 - (T)1) / tk;
+        hipLaunchKernelGGL((k_fgp_L<S, T>), gr, b, 0, ctx->stream, (const S *)vol, r, s, t, (S *)nullptr, L, M, N, Z, lam);
+        hipLaunchKernelGGL((k_fgp_LT<T>), gr, b, 0, ctx->stream, r, s, t, (const T *)L, o, p, q, M, N, Z, g, a12);
+        tk1 = tk;
+    }
+    hipLaunchKernelGGL((k_fgp_L<S, T>), gr, b, 0, ctx->stream, (const S *)vol, o, p, q, (S *)vol, (T *)nullptr, M, N, Z, lam);
+    return launch_check(ctx, "fgp");
+}
+
+ct_status launch_fgp(ct_context *ctx, int M, int N, int Z, void *vol, ct_dtype dt, double lambda, int niter) {
+    switch (dt) {
+    case CT_F32: return run_fgp<float, float>(ctx, M, N, Z, vol, lambda, niter);
+    case CT_F16: return run_fgp<__half, float>(ctx, M, N, Z, vol, lambda, niter);
+    case CT_BF16: return run_fgp<__hip_bfloat16, float>(ctx, M, N, Z, vol, lambda, niter);
+    case CT_F64: return run_fgp<double, double>(ctx, M, N, Z, vol, lambda, niter);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "fgp: bad dtype");
+}
+
+// D: x -> forward differences (zero on the last plane); D^T: divergence form.
+template <typename S>
+__global__ void k_fgp_D(const S *x, S *r, S *s, S *t, int M, int N, int Z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+    const double c = ld64(x, v);
+    double a = i < M - 1 ? c - ld64(x, v + 1) : 0.0;
+    double bb = j < N - 1 ? c - ld64(x, v + M) : 0.0;
+    double e = k < Z - 1 ? c - ld64(x, v + MN) : 0.0;
+    if constexpr (std::is_same<S, double>::value) { r[v] = a; s[v] = bb; t[v] = e; }
+    else { St<S>::st(r, v, (float)a); St<S>::st(s, v, (float)bb); St<S>::st(t, v, (float)e); }
+}
+template <typename S>
+__global__ void k_fgp_Dt(const S *r, const S *s, const S *t, S *out, int M, int N, int Z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+    double rv = i < M - 1 ? ld64(r, v) : 0.0, sv = j < N - 1 ? ld64(s, v) : 0.0, tv = k < Z - 1 ? ld64(t, v) : 0.0;
+    double rm = i > 0 ? ld64(r, v - 1) : 0.0, sm = j > 0 ? ld64(s, v - M) : 0.0, tm = k > 0 ? ld64(t, v - MN) : 0.0;
+    double o = (rv - rm) + (sv - sm) + (tv - tm);
+    if constexpr (std::is_same<S, double>::value) out[v] = o; else St<S>::st(out, v, (float)o);
+}
+
+ct_status launch_fgp_D(ct_context *ctx, int M, int N, int Z, const void *x, void *r, void *s, void *t, ct_dtype dt) {
+    dim3 b(256), gr((M + 255) / 256, N, Z);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_fgp_D<float>, gr, b, 0, ctx->stream, (const float *)x, (float *)r, (float *)s, (float *)t, M, N, Z); break;
+    case CT_F16: hipLaunchKernelGGL(k_fgp_D<__half>, gr, b, 0, ctx->stream, (const __half *)x, (__half *)r, (__half *)s, (__half *)t, M, N, Z); break;
+    case CT_BF16: hipLaunchKernelGGL(k_fgp_D<__hip_bfloat16>, gr, b, 0, ctx->stream, (const __hip_bfloat16 *)x, (__hip_bfloat16 *)r, (__hip_bfloat16 *)s, (__hip_bfloat16 *)t, M, N, Z); break;
+    case CT_F64: hipLaunchKernelGGL(k_fgp_D<double>, gr, b, 0, ctx->stream, (const double *)x, (double *)r, (double *)s, (double *)t, M, N, Z); break;
+    }
+    return launch_check(ctx, "fgp D");
+}
+ct_status launch_fgp_Dt(ct_context *ctx, int M, int N, int Z, const void *r, const void *s, const void *t, void *out, ct_dtype dt) {
+    dim3 b(256), gr((M + 255) / 256, N, Z);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL(k_fgp_Dt<float>, gr, b, 0, ctx->stream, (const float *)r, (const float *)s, (const float *)t, (float *)out, M, N, Z); break;
+    case CT_F16: hipLaunchKernelGGL(k_fgp_Dt<__half>, gr, b, 0, ctx->stream, (const __half *)r, (const __half *)s, (const __half *)t, (__half *)out, M, N, Z); break;
+    case CT_BF16: hipLaunchKernelGGL(k_fgp_Dt<__hip_bfloat16>, gr, b, 0, ctx->stream, (const __hip_bfloat16 *)r, (const __hip_bfloat16 *)s, (const __hip_bfloat16 *)t, (__hip_bfloat16 *)out, M, N, Z); break;
+    case CT_F64: hipLaunchKernelGGL(k_fgp_Dt<double>, gr, b, 0, ctx->stream, (const double *)r, (const double *)s, (const double *)t, (double *)out, M, N, Z); break;
+    }
+    return launch_check(ctx, "fgp Dt");
+}
+
+} // namespace ct

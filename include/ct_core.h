@@ -1,0 +1,277 @@
+/*
+ * ct_core.h - C ABI of the MI355X (gfx950) cone-beam CT reconstruction core.
+ *
+ * Every entry point takes DEVICE pointers (HBM-resident buffers owned by the
+ * caller), runs asynchronously on the context's HIP stream unless it returns a
+ * host scalar, and returns a ct_status.  No entry point exits the process or
+ * stages host memory implicitly.  Host-array wrappers with the reference's own
+ * function names are declared in ct_compat.h.
+ *
+ * Reference call surface replaced (paths relative to the reference root,
+ * V63 = CBCT/CBCT_1024V_1024P_multiGPU_v6.3_20170823,
+ * TB  = TBCT/TBCT_1024V_1024P_v4.2_20171012):
+ *   compile-time geometry            V63/main_recon.cu:54-111, TB/main_recon.cu:83-117
+ *   Forward_3D_ray_driven_siddon     V63/host_IterativeRecon_CBCT.c:70
+ *   Reconstruction_3D_ray_driven_CBCT V63/host_IterativeRecon_CBCT.c:1
+ *   CheckMatchedJointOperator        V63/host_IterativeRecon_CBCT.c:131
+ *   pixel-driven (matched) SART      V63/pixel_driven_backprj/host_IterativeRecon_CBCT.c:1
+ *   FISTA momentum (inline)          V63/main_recon.cu:372-384
+ *   FGP_denoise_GPUx7_exact          V63/host_FGP_Denoise_GPUx7.c:1
+ *   Reconstruction_FDK               V63/host_FDK.cpp:4
+ *   RampFiltering_GPU                V63/FDK/FFT_1d_GPU.h:241
+ *   L2_norm_gpu / TV_norm_gpu        V63/host_tool_functions.cu:28,61
+ *   tv_gradient_matrix_3d_kernel     V63/kernel_IterativeRecon_universal.cu:912
+ *   Reconstruction_3D_ray_driven_TBCT TB/voxel/host_IterativeRecon_TBCT_voxel.c:1
+ *   Forward_3D_ray_driven_siddon_TBCT TB/voxel/host_IterativeRecon_TBCT_voxel.c:111
+ */
+#ifndef CT_CORE_H
+#define CT_CORE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CT_ABI_VERSION 1
+
+typedef enum ct_status {
+    CT_OK = 0,
+    CT_ERR_INVALID_ARG = 1,   /* bad scalar argument (beta, niter, view range ...) */
+    CT_ERR_GEOMETRY = 2,      /* geometry fails validation */
+    CT_ERR_DTYPE = 3,         /* unsupported dtype or dtype combination */
+    CT_ERR_NULL = 4,          /* null pointer */
+    CT_ERR_ALIGN = 5,         /* pointer not aligned to its element size */
+    CT_ERR_OVERFLOW = 6,      /* a size exceeds the kernels' 32-bit index range */
+    CT_ERR_ALLOC = 7,         /* device scratch allocation failed */
+    CT_ERR_DEVICE = 8,        /* no usable HIP device / HIP runtime error */
+    CT_ERR_LAUNCH = 9         /* kernel launch failed */
+} ct_status;
+
+/* Storage types.  Arithmetic is FP32 for F32/F16/BF16 storage and FP64 for
+ * F64 storage; ray sums and backprojection sums are always accumulated in the
+ * arithmetic type, never in the storage type. */
+typedef enum ct_dtype {
+    CT_F32 = 0,
+    CT_F16 = 1,
+    CT_BF16 = 2,
+    CT_F64 = 3
+} ct_dtype;
+
+/* Runtime replacement of the reference's compile-time constants
+ * (V63/main_recon.cu:54-111).  Lengths are in any one unit (the reference uses
+ * metres, the synthetic workloads millimetres); every threshold of the
+ * algorithms scales with the voxel pitch, so the unit does not matter.
+ *
+ * Coordinates: volume index f[k*ny*nx + j*nx + i] (x fastest), voxel centre
+ *   x_i = bx + (i+0.5)*vx with bx = -vx*(nx/2 + x_offset), by = -vy*ny/2,
+ *   bz = -vz*nz/2.
+ * Detector: ncols x nrows, stored [row][col]; column centre
+ *   u_c = -du*(ncols/2 - 0.5) + det_offset + c*du, row centre
+ *   v_r = -dv*(nrows/2 - 0.5) + r*dv.  The reference uses one pitch for both.
+ * Source (view angle phi): (dso*cos - det_offset*sin, dso*sin + det_offset*cos, z_s)
+ *   with z_s = src_z0 + s*src_dz for source s; detector pixel at
+ *   (dod*cos - u*sin, dod*sin + u*cos, v).  dod < 0 puts the detector across
+ *   the isocentre (SDD = dso - dod).
+ * Angles: phi_j = -((float)(j*angle_step_deg + angle0_deg) + angle_shift_deg)*pi/180
+ *   for the iterative operators (V63/host_IterativeRecon_CBCT.c:34-36) and +theta
+ *   for FDK (V63/host_FDK.cpp:35-37).
+ * nsources > 1 selects the multi-source (TB) variants of the SART kernels:
+ *   projection stacks are [view][source][row][col].
+ */
+typedef struct ct_geometry {
+    int32_t nx, ny, nz;           /* M, N, ZETA */
+    int32_t x_offset;             /* M_Offset (half-detector use) */
+    double vx, vy, vz;            /* volumn_x, volumn_y, volumn_z */
+    int32_t ncols, nrows;         /* R, Z_prj */
+    double du, dv;                /* Detector_pixel_x (columns, rows) */
+    double det_offset;            /* Offset (= Source_y) */
+    double dso, dod;              /* DSO, DOD */
+    int32_t nviews;               /* Nviews */
+    int32_t nsources;             /* 1 for CBCT; N_source for TB */
+    double angle_step_deg;        /* us_rate */
+    double angle0_deg;            /* initialAngle */
+    double angle_shift_deg;       /* shiftAngle */
+    double src_z0, src_dz;        /* Source_z (CBCT) / Source_z_min, Source_interval (TB) */
+} ct_geometry;
+
+/* Ramp-filter window for FDK. */
+typedef enum ct_window {
+    CT_WINDOW_RAMP = 0,   /* pure ramp x 0.5, as V63/FDK/FFT_1d_GPU.h:256-267 */
+    CT_WINDOW_HANN = 1    /* ramp x 0.5 x 0.5*(1+cos(2*pi*k/pad)), k in [-pad/2, pad/2) */
+} ct_window;
+
+/* View order inside a SART sweep. */
+typedef enum ct_order {
+    CT_ORDER_ASCENDING = 0,   /* subset s = views {s*S, ..., s*S+S-1}: S=1 is the reference order */
+    CT_ORDER_INTERLEAVED = 1  /* subset s = views {s, s+n_sub, s+2*n_sub, ...} */
+} ct_order;
+
+typedef struct ct_context ct_context;
+
+/* ---- context --------------------------------------------------------- */
+int         ct_abi_version(void);
+ct_status   ct_create(ct_context **ctx, int device);
+ct_status   ct_destroy(ct_context *ctx);
+/* Use a caller-owned hipStream_t (NULL = the legacy default stream). */
+ct_status   ct_set_stream(ct_context *ctx, void *hip_stream);
+ct_status   ct_synchronize(ct_context *ctx);
+const char *ct_last_error(const ct_context *ctx);
+const char *ct_status_string(ct_status s);
+/* Validates a geometry without touching the GPU. */
+ct_status   ct_validate_geometry(const ct_geometry *g);
+/* Per-view angle (radians) exactly as the kernels use it; fdk selects +theta. */
+ct_status   ct_view_angle(const ct_geometry *g, int32_t view, int32_t fdk,
+                          float *sin_out, float *cos_out);
+/* Bytes of device scratch the context currently holds. */
+size_t      ct_scratch_bytes(const ct_context *ctx);
+
+/* ---- projector pair ---------------------------------------------------- */
+/* Siddon ray sums (command 0 of forward_ray_driven_3d_kernel_correction,
+ * V63/kernel_IterativeRecon_universal.cu:7-567) for views [view0, view0+nviews):
+ * proj[(v-view0)][src][row][col]. */
+ct_status ct_forward(ct_context *ctx, const ct_geometry *g,
+                     const void *vol, ct_dtype vol_dtype,
+                     void *proj, ct_dtype proj_dtype,
+                     int32_t view0, int32_t nviews);
+
+/* SART correction (command 1, same kernel): corr = (b - sum l*f)/sum l, 0 where
+ * sum l < 1e-6*vz.  meas holds the views' measured projections; corr is F32
+ * (F64 when vol_dtype is F64). */
+ct_status ct_sart_correction(ct_context *ctx, const ct_geometry *g,
+                             const void *vol, ct_dtype vol_dtype,
+                             const void *meas, ct_dtype meas_dtype,
+                             void *corr,
+                             int32_t view0, int32_t nviews);
+
+/* Voxel-driven backprojection (backprj_ray_driven_3d_kernel,
+ * V63/kernel_IterativeRecon_universal.cu:571-810), summed over the nviews
+ * correction images: mode 1: x += beta*W/L where L >= 1e-6*vx (SART update);
+ * mode 0: x = W where L >= 1e-6*vx (unchanged elsewhere). */
+ct_status ct_backproject_voxel(ct_context *ctx, const ct_geometry *g,
+                               void *vol, ct_dtype vol_dtype,
+                               const void *corr,
+                               int32_t view0, int32_t nviews,
+                               double beta, int32_t mode);
+
+/* One SART sweep over all views, device resident (Reconstruction_3D_ray_driven_CBCT,
+ * V63/host_IterativeRecon_CBCT.c:1-67).  subset_size = 1 with CT_ORDER_ASCENDING
+ * reproduces the reference's view-by-view order; larger subsets accumulate W and
+ * L over the subset's views before one update. */
+ct_status ct_sart_sweep(ct_context *ctx, const ct_geometry *g,
+                        void *vol, ct_dtype vol_dtype,
+                        const void *meas, ct_dtype meas_dtype,
+                        double beta, int32_t subset_size, ct_order order);
+
+/* Matched adjoint: vol = A^T proj, the exact transpose of ct_forward's Siddon
+ * segment lengths (scatter form of V63/pixel_driven_backprj/kernel_IterativeRecon_CBCT.cu:563-1116,
+ * with the forward's 1e-16 loop tolerance and ray-length cut so that the pair
+ * is transposed exactly).  Overwrites vol. */
+ct_status ct_adjoint(ct_context *ctx, const ct_geometry *g,
+                     const void *proj, ct_dtype proj_dtype,
+                     void *vol, ct_dtype vol_dtype,
+                     int32_t view0, int32_t nviews);
+
+/* <A x, y> and <x, A^T y> in FP64 for seeded x, y ~ N(0,1) over all views
+ * (CheckMatchedJointOperator, V63/host_IterativeRecon_CBCT.c:131-204). */
+ct_status ct_adjoint_test(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
+                          uint64_t seed, double *lhs, double *rhs);
+
+/* Largest eigenvalue of A^T A by niter power iterations (FP64 norms). */
+ct_status ct_power_iteration(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
+                             int32_t niter, uint64_t seed, double *lipschitz);
+
+/* ---- FISTA / TV ------------------------------------------------------- */
+/* y = x + gamma*(x - x_prev); x_prev = x (V63/main_recon.cu:375-382).  y may
+ * alias x, which fuses the reference's F_recon <- F_Y_k copy away. */
+ct_status ct_fista_momentum(ct_context *ctx, void *x, void *x_prev, void *y,
+                            int64_t n, ct_dtype dtype, double gamma);
+
+/* FISTA scalar t_k = (1 + sqrt(1 + 4 t_{k-1}^2))/2 and gamma = (t_{k-1}-1)/t_k in FP64. */
+ct_status ct_fista_gamma(double t_prev, double *t_next, double *gamma);
+
+/* TV-FGP denoise in place (FGP_denoise_GPUx7_exact, V63/host_FGP_Denoise_GPUx7.c:1-111
+ * and kernels V63/kernel_FGP_Denoise_GPUx7.cu:2-88): x* = argmin ||x-b||^2 + 2*lambda*TV(x).
+ * lambda == 0 leaves vol unchanged. */
+ct_status ct_tv_fgp(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
+                    void *vol, ct_dtype dtype, double lambda, int32_t niter);
+
+/* The linear pair inside FGP: D maps a volume to its three forward differences
+ * (r,s,t)_v = (x_v - x_{v+e}) where v+e is inside, else 0 (the L^T operator,
+ * V63/kernel_FGP_Denoise_GPUx7.cu:58-65); D^T maps (r,s,t) back:
+ * (D^T p)_v = p_v - p_{v-e} summed over axes (the L operator, :14).  Dual
+ * fields are volume-sized arrays, one per axis. */
+ct_status ct_tv_fgp_adjoint(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
+                            const void *p_r, const void *p_s, const void *p_t,
+                            void *out, ct_dtype dtype);
+ct_status ct_tv_fgp_forward_op(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
+                               const void *x, void *p_r, void *p_s, void *p_t,
+                               ct_dtype dtype);
+
+/* TV_eps(x) = sum_v sqrt(eps + sum_d (x_v - x_{v-e_d})^2) over the backward
+ * differences that exist inside the volume (the definition of
+ * tv_gradient_matrix_3d_kernel, V63/kernel_IterativeRecon_universal.cu:912-949,
+ * extended to the boundary with missing neighbours dropped).  FP64 result. */
+ct_status ct_tv_eps(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
+                    const void *vol, ct_dtype dtype, double eps, double *value);
+ct_status ct_tv_eps_grad(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
+                         const void *vol, ct_dtype dtype, double eps, void *grad);
+
+/* ---- norms (FP64 on the GPU) ------------------------------------------ */
+/* sum (a-b)^2 (L2_norm_gpu, V63/host_tool_functions.cu:28-58, without the
+ * overwrite of a). */
+ct_status ct_residual_norm2(ct_context *ctx, const void *a, const void *b,
+                            int64_t n, ct_dtype dtype, double *value);
+/* Isotropic forward-difference TV (TV_norm_kernel, V63/kernel_tool_functions.cu:39-65). */
+ct_status ct_tv_norm(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
+                     const void *vol, ct_dtype dtype, double *value);
+/* sum a*b in FP64. */
+ct_status ct_dot(ct_context *ctx, const void *a, const void *b, int64_t n,
+                 ct_dtype dtype, double *value);
+
+/* ---- FDK -------------------------------------------------------------- */
+/* Cosine weight (kernel_FDK_projection_adjust, V63/FDK/kernel_FDK.cu:1-14) of
+ * all views, out-of-place allowed in == out. */
+ct_status ct_fdk_weight(ct_context *ctx, const ct_geometry *g,
+                        const void *proj_in, void *proj_out, ct_dtype dtype);
+/* Row ramp filter by a zero-padded 2*ncols FFT (RampFiltering_GPU,
+ * V63/FDK/FFT_1d_GPU.h:241-287) of nimg images of nrows rows. */
+ct_status ct_ramp_filter(ct_context *ctx, const ct_geometry *g,
+                         const void *in, void *out, ct_dtype dtype,
+                         int32_t nimg, ct_window window);
+/* Weighted bilinear backprojection of filtered views [view0, view0+nviews)
+ * accumulated into vol (kernel_FDK_reconstruction, V63/FDK/kernel_FDK.cu:16-62). */
+ct_status ct_fdk_backproject(ct_context *ctx, const ct_geometry *g,
+                             const void *filtered, ct_dtype proj_dtype,
+                             void *vol, ct_dtype vol_dtype,
+                             int32_t view0, int32_t nviews);
+/* Whole FDK (Reconstruction_FDK, V63/host_FDK.cpp:4-57): vol += FDK(proj).
+ * Scratch for the filtered stack is held by the context. */
+ct_status ct_fdk(ct_context *ctx, const ct_geometry *g,
+                 const void *proj, ct_dtype proj_dtype,
+                 void *vol, ct_dtype vol_dtype, ct_window window);
+
+/* ---- driver ----------------------------------------------------------- */
+/* One outer iteration of SART + TV-FGP + FISTA momentum (V63/main_recon.cu:344-385),
+ * device resident.  On entry x holds y_k (the extrapolated point, initially 0)
+ * and x_prev holds x_{k-1}; on return x holds y_{k+1} and x_prev holds x_k.
+ * *t_state carries t_{k-1} (1.0 before the first iteration).  fgp_lambda <= 0
+ * skips the TV step. */
+ct_status ct_sart_fista_iteration(ct_context *ctx, const ct_geometry *g,
+                                  void *x, void *x_prev, ct_dtype vol_dtype,
+                                  const void *meas, ct_dtype meas_dtype,
+                                  double beta, int32_t subset_size, ct_order order,
+                                  double fgp_lambda, int32_t fgp_niter,
+                                  double *t_state);
+
+/* ---- utilities -------------------------------------------------------- */
+/* Element conversion between storage types (n elements, device). */
+ct_status ct_convert(ct_context *ctx, const void *src, ct_dtype src_dtype,
+                     void *dst, ct_dtype dst_dtype, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CT_CORE_H */

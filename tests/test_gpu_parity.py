@@ -1,0 +1,345 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bars: plain FP32 path <= 1e-5 relative L2 on
+projections and <= 1e-4 on reconstructions; FP64 paths are held to 1e-12;
+the matched adjoint passes the dot-product test to 1e-12 (FP64) / 1e-6 (FP32).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_amd import core
+from gpu_amd.geometry import Geometry, adjoint_64, plain_sart_128
+from gpu_amd.phantom import phantom
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def dev(x, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(x)).to(device="cuda", dtype=dtype).contiguous()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().to("cpu", torch.float64).numpy()
+
+
+def small_geom(**kw):
+    base = dict(nx=24, ny=20, nz=16, vx=1.0, ncols=40, nrows=28, du=1.1, nviews=9, dso=90.0, dod=-45.0,
+                angle_step_deg=37.3)
+    base.update(kw)
+    return Geometry(**base)
+
+
+GEOMS = {
+    "small": small_geom(),
+    "offsets": small_geom(x_offset=3, det_offset=2.2, ncols=37, nrows=21),
+    "aniso": small_geom(nx=17, ny=23, nz=11, vx=0.9, vy=1.2, vz=0.7, du=0.8, dv=1.3),
+    "adjoint_64": adjoint_64().with_views(15),
+    "multisource": small_geom(nsources=4, src_z0=-9.0, src_dz=6.0, nrows=12),
+}
+
+
+@pytest.mark.parametrize("name", list(GEOMS))
+def test_forward_fp32(ctx, name):
+    g = GEOMS[name]
+    x = phantom(g, seed=11, scale=0.3).numpy()
+    ref = orc.forward(g, x, prec=32)
+    got = host(ctx.forward(g, dev(x)))
+    r = rel(got, ref)
+    print(f"{name}: forward FP32 rel L2 {r:.3e}, max abs {np.abs(got - ref).max():.3e}")
+    assert r <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["small", "aniso", "multisource"])
+def test_forward_fp64(ctx, name):
+    g = GEOMS[name]
+    x = phantom(g, seed=12, scale=0.3).numpy().astype(np.float64)
+    ref = orc.forward(g, x, prec=64)
+    got = host(ctx.forward(g, dev(x, torch.float64)))
+    assert rel(got, ref) <= 1e-12
+
+
+def test_forward_plain_sart_geometry_view_subset(ctx):
+    g = plain_sart_128()
+    x = phantom(g, seed=0).numpy()
+    for v0 in (0, 44, 123):
+        ref = orc.forward(g, x, prec=32, view0=v0, nviews=3)
+        got = host(ctx.forward(g, dev(x), view0=v0, nviews=3))
+        r = rel(got, ref)
+        print(f"view0 {v0}: rel {r:.3e}")
+        assert r <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["small", "offsets", "multisource"])
+def test_sart_correction(ctx, name):
+    g = GEOMS[name]
+    x = phantom(g, seed=13, scale=0.3).numpy()
+    meas = orc.forward(g, x, prec=32, view0=3, nviews=2) * 1.1 + 0.01
+    ref = orc.sart_correction(g, x * 0.9, meas, view0=3, nviews=2, prec=32)
+    got = host(ctx.sart_correction(g, dev(x * 0.9), dev(meas), view0=3, nviews=2))
+    assert rel(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource"])
+@pytest.mark.parametrize("mode,nv", [(1, 1), (0, 1), (1, 4)])
+def test_backproject_voxel(ctx, name, mode, nv):
+    g = GEOMS[name]
+    rng = np.random.default_rng(5)
+    x = phantom(g, seed=14, scale=0.3).numpy()
+    corr = rng.standard_normal((nv, *g.proj_shape[1:])).astype(np.float32) * 0.01
+    ref = orc.backproject_voxel(g, x, corr, view0=2, nviews=nv, beta=0.7, mode=mode, prec=32)
+    v = dev(x)
+    ctx.backproject_voxel(g, v, dev(corr), view0=2, nviews=nv, beta=0.7, mode=mode)
+    got = host(v)
+    r = rel(got, ref)
+    print(f"{name} mode {mode} nv {nv}: rel {r:.3e} max {np.abs(got - ref).max():.3e}")
+    assert r <= 1e-5
+
+
+def test_backproject_voxel_fp64(ctx):
+    g = GEOMS["aniso"]
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(g.vol_shape)
+    corr = rng.standard_normal((3, *g.proj_shape[1:]))
+    ref = orc.backproject_voxel(g, x, corr, view0=1, nviews=3, beta=0.3, mode=1, prec=64)
+    v = dev(x, torch.float64)
+    ctx.backproject_voxel(g, v, dev(corr, torch.float64), view0=1, nviews=3, beta=0.3, mode=1)
+    assert rel(host(v), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("name,subset,order", [("small", 1, "ascending"), ("offsets", 3, "interleaved"),
+                                               ("multisource", 1, "ascending"), ("aniso", 4, "ascending")])
+def test_sart_sweep(ctx, name, subset, order):
+    g = GEOMS[name]
+    x = phantom(g, seed=15, scale=0.3).numpy()
+    b = orc.forward(g, x, prec=32)
+    ref = np.zeros_like(x)
+    v = torch.zeros(g.vol_shape, device="cuda")
+    bd = dev(b)
+    for _ in range(2):
+        ref = orc.sart_sweep(g, ref, b, beta=0.9, subset=subset, order=order, prec=32)
+        ctx.sart_sweep(g, v, bd, beta=0.9, subset=subset, order=order)
+    r = rel(host(v), ref)
+    print(f"{name}: SART rel {r:.3e}")
+    assert r <= 1e-4
+
+
+def test_sart_plain_128_two_iterations(ctx):
+    """128^3 / 180 views / 256^2, relaxation 1.0, subsets of 20 views (interleaved),
+    2 iterations against the FP32 oracle; error vs the FP64 oracle reported."""
+    g = plain_sart_128()
+    x = phantom(g, seed=0).numpy()
+    b = orc.forward(g, x, prec=32)
+    v = torch.zeros(g.vol_shape, device="cuda")
+    bd = dev(b)
+    ref = np.zeros_like(x)
+    for _ in range(2):
+        ctx.sart_sweep(g, v, bd, beta=1.0, subset=20, order="interleaved")
+        ref = orc.sart_sweep(g, ref, b, beta=1.0, subset=20, order="interleaved", prec=32)
+    got = host(v)
+    r = rel(got, ref)
+    print(f"plain_sart_128: GPU vs FP32 oracle rel {r:.3e}; vs truth rel {rel(got, x):.3e}")
+    assert r <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["small", "multisource"])
+def test_adjoint_matches_oracle(ctx, name):
+    g = GEOMS[name]
+    rng = np.random.default_rng(8)
+    y = rng.standard_normal(g.proj_shape)
+    ref64 = orc.adjoint(g, y, prec=64)
+    got64 = host(ctx.adjoint(g, dev(y, torch.float64)))
+    assert rel(got64, ref64) <= 1e-12
+    ref32 = orc.adjoint(g, y, prec=32)
+    got32 = host(ctx.adjoint(g, dev(y)))
+    assert rel(got32, ref32) <= 1e-5
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-6)])
+def test_adjoint_dot_product_adjoint_64(ctx, dtype, tol):
+    g = adjoint_64()
+    lhs, rhs = ctx.adjoint_test(g, dtype=dtype, seed=2024)
+    err = abs(lhs - rhs) / abs(lhs)
+    print(f"{dtype}: <Ax,y>={lhs:.12e} <x,A^T y>={rhs:.12e} rel {err:.3e}")
+    assert err <= tol
+
+
+def test_power_iteration_vs_dense_svd(ctx):
+    g = Geometry(nx=6, ny=6, nz=4, vx=1.0, ncols=9, nrows=6, du=1.2, nviews=5, dso=40.0, dod=-20.0)
+    n = g.nvox
+    A = np.zeros((g.rays, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        A[:, i] = orc.forward(g, e.reshape(g.vol_shape), prec=64).ravel()
+    smax2 = np.linalg.svd(A, compute_uv=False)[0] ** 2
+    lip = ctx.power_iteration(g, dtype=torch.float64, niter=300, seed=1)
+    assert lip == pytest.approx(smax2, rel=1e-6)
+
+
+def test_fista_momentum(ctx):
+    rng = np.random.default_rng(1)
+    x, xp = rng.standard_normal((8, 9, 10)), rng.standard_normal((8, 9, 10))
+    tn, gm = core.Context.fista_gamma(2.5)
+    assert tn == pytest.approx((1 + math.sqrt(1 + 4 * 6.25)) / 2)
+    for dt, prec in ((torch.float32, 32), (torch.float64, 64)):
+        X, P = dev(x, dt), dev(xp, dt)
+        ctx.fista_momentum(X, P, X, gm)
+        y_ref, p_ref = orc.fista_momentum(x, xp, gm, prec=prec)
+        assert rel(host(X), y_ref) <= (1e-7 if prec == 32 else 1e-15)
+        assert rel(host(P), p_ref) == 0.0
+
+
+@pytest.mark.parametrize("shape", [(7, 9, 11), (16, 24, 32)])
+def test_tv_fgp(ctx, shape):
+    rng = np.random.default_rng(4)
+    b = (rng.standard_normal(shape) * 0.01 + 0.02).astype(np.float32)
+    ref = orc.tv_fgp(b, 0.004, 20, prec=32)
+    v = dev(b)
+    ctx.tv_fgp(v, 0.004, 20)
+    r = rel(host(v), ref)
+    print(f"FGP {shape}: rel {r:.3e}")
+    assert r <= 1e-5
+    ref64 = orc.tv_fgp(b.astype(np.float64), 0.004, 20, prec=64)
+    v = dev(b, torch.float64)
+    ctx.tv_fgp(v, 0.004, 20)
+    assert rel(host(v), ref64) <= 1e-12
+
+
+def test_tv_difference_pair(ctx):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((6, 7, 8))
+    r, s, t = ctx.tv_fgp_D(dev(x, torch.float64))
+    rr, ss, tt = orc.tv_D(x)
+    for a, b in ((r, rr), (s, ss), (t, tt)):
+        assert rel(host(a), b) <= 1e-15
+    p = [rng.standard_normal((6, 7, 8)) for _ in range(3)]
+    out = host(ctx.tv_fgp_Dt(*(dev(q, torch.float64) for q in p)))
+    assert rel(out, orc.tv_Dt(*p)) <= 1e-15
+
+
+def test_norms(ctx):
+    rng = np.random.default_rng(6)
+    a, b = rng.standard_normal((9, 10, 11)), rng.standard_normal((9, 10, 11))
+    for dt, prec, tol in ((torch.float64, 64, 1e-13), (torch.float32, 32, 1e-12)):
+        A, B = dev(a, dt), dev(b, dt)
+        assert ctx.residual_norm2(A, B) == pytest.approx(orc.residual_norm2(host(A), host(B), prec=64), rel=tol)
+        assert ctx.tv_norm(A) == pytest.approx(orc.tv_norm(host(A), prec=64), rel=tol)
+        assert ctx.tv_eps(A, 1e-8) == pytest.approx(orc.tv_eps(host(A), 1e-8), rel=tol)
+        assert ctx.dot(A, B) == pytest.approx(float((host(A) * host(B)).sum()), rel=1e-11)
+
+
+def test_tv_eps_grad(ctx):
+    x = np.random.default_rng(7).standard_normal((5, 6, 7)) * 0.1
+    got = host(ctx.tv_eps_grad(dev(x, torch.float64), 1e-8))
+    assert rel(got, orc.tv_eps_grad(x, 1e-8)) <= 1e-12
+
+
+FDK_G = Geometry(nx=48, ny=48, nz=24, vx=1.0, ncols=96, nrows=40, du=1.0, nviews=60, dso=300.0, dod=-150.0)
+
+
+@pytest.mark.parametrize("window", ["ramp", "hann"])
+def test_fdk_stages(ctx, window):
+    g = FDK_G
+    x = phantom(g, seed=3, scale=0.4).numpy()
+    p = orc.forward(g, x, prec=32)
+    w_ref = orc.fdk_weight(g, p, prec=32)
+    w = ctx.fdk_weight(g, dev(p))
+    assert rel(host(w), w_ref) <= 1e-6
+    f_ref = orc.ramp_filter(g, w_ref, window, prec=32)
+    f = ctx.ramp_filter(g, dev(w_ref), window=window)
+    r = rel(host(f), f_ref)
+    print(f"ramp {window}: rel {r:.3e}")
+    assert r <= 1e-5
+    v_ref = orc.fdk_backproject(g, f_ref, np.zeros(g.vol_shape, np.float32), prec=32)
+    v = ctx.fdk_backproject(g, dev(f_ref), torch.zeros(g.vol_shape, device="cuda"))
+    r = rel(host(v), v_ref)
+    print(f"fdk backprojection: rel {r:.3e}")
+    assert r <= 1e-5
+    full_ref = orc.fdk(g, p, window, prec=32)
+    full = ctx.fdk(g, dev(p), window=window)
+    r = rel(host(full), full_ref)
+    print(f"fdk {window}: rel {r:.3e}")
+    assert r <= 1e-4
+
+
+def test_ramp_filter_non_power_of_two(ctx):
+    g = Geometry(nx=8, ny=8, nz=8, vx=1.0, ncols=45, nrows=7, du=0.9, nviews=3, dso=100.0, dod=-50.0)
+    p = np.random.default_rng(2).standard_normal((3, 7, 45)).astype(np.float32)
+    ref = orc.ramp_filter(g, p, "hann", prec=64)
+    got = host(ctx.ramp_filter(g, dev(p), window="hann"))
+    assert rel(got, ref) <= 1e-5
+
+
+def test_sart_fista_iteration_matches_composition(ctx):
+    g = GEOMS["small"]
+    x = phantom(g, seed=16, scale=0.3).numpy()
+    b = orc.forward(g, x, prec=32)
+    X = torch.zeros(g.vol_shape, device="cuda")
+    XP = torch.zeros_like(X)
+    y_ref, xp_ref, t = np.zeros_like(x), np.zeros_like(x), 1.0
+    bd = dev(b)
+    t_gpu = 1.0
+    for _ in range(3):
+        t_gpu = ctx.sart_fista_iteration(g, X, XP, bd, t_gpu, beta=0.8, subset=3, order="interleaved",
+                                         fgp_lambda=0.002, fgp_niter=10)
+        xk = orc.sart_sweep(g, y_ref, b, beta=0.8, subset=3, order="interleaved", prec=32)
+        xk = orc.tv_fgp(xk, 0.002, 10, prec=32)
+        tn = (1 + math.sqrt(1 + 4 * t * t)) / 2
+        y_ref, xp_ref = orc.fista_momentum(xk, xp_ref, (t - 1) / tn, prec=32)
+        t = tn
+    assert t_gpu == pytest.approx(t)
+    assert rel(host(XP), xp_ref) <= 1e-4
+    assert rel(host(X), y_ref) <= 1e-4
+
+
+@pytest.mark.parametrize("dt", [torch.float16, torch.bfloat16])
+def test_reduced_precision_storage(ctx, dt):
+    g = GEOMS["small"]
+    x = phantom(g, seed=17, scale=0.3).numpy()
+    xs = torch.as_tensor(x).to(dt)
+    x_rounded = xs.float().numpy()
+    ref = orc.forward(g, x_rounded, prec=32)
+    got = host(ctx.forward(g, xs.cuda(), out_dtype=torch.float32))
+    assert rel(got, ref) <= 1e-6           # FP32 accumulation of the stored values
+    got16 = host(ctx.forward(g, xs.cuda()))
+    assert rel(got16, ref) <= (1e-3 if dt == torch.float16 else 8e-3)
+
+
+def test_compat_layer(ctx):
+    lib = core.lib()
+    g = GEOMS["small"]
+    gc = g.to_c()
+    lib.ct_compat_set_geometry.argtypes = [ctypes.POINTER(type(gc))]
+    lib.ct_compat_set_geometry.restype = ctypes.c_int
+    assert lib.ct_compat_set_geometry(ctypes.byref(gc)) == 0
+    x = phantom(g, seed=18, scale=0.3).numpy().astype(np.float32)
+    proj = np.zeros(g.proj_shape, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.Forward_3D_ray_driven_siddon.argtypes = [fp, fp]
+    lib.Forward_3D_ray_driven_siddon(x.ctypes.data_as(fp), proj.ctypes.data_as(fp))
+    assert lib.ct_compat_last_status() == 0
+    assert rel(proj, orc.forward(g, x, prec=32)) <= 1e-5
+    vol = np.zeros_like(x)
+    lib.Reconstruction_3D_ray_driven_CBCT.argtypes = [fp, fp, ctypes.c_float]
+    lib.Reconstruction_3D_ray_driven_CBCT(vol.ctypes.data_as(fp), proj.ctypes.data_as(fp), ctypes.c_float(0.5))
+    assert rel(vol, orc.sart_sweep(g, np.zeros_like(x), proj, beta=0.5, subset=1, prec=32)) <= 1e-4
+    a = proj.copy()
+    b = proj * 0.5
+    lib.L2_norm_gpu.argtypes = [fp, fp]
+    lib.L2_norm_gpu.restype = ctypes.c_double
+    v = lib.L2_norm_gpu(a.ctypes.data_as(fp), b.ctypes.data_as(fp))
+    assert v == pytest.approx(float(((proj.astype(np.float64) * 0.5) ** 2).sum()), rel=1e-6)
+    np.testing.assert_allclose(a, (proj - b) ** 2, rtol=1e-6)
+    lib.ct_compat_release()
