@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: fwd+back GRays/s and seconds per SART-FISTA-TV iteration at
+512^3 voxels / 1024 views of 1024^2, FP32 storage vs FP16 storage.
+
+Workload (synthetic, seeded): 512^3 volume of 0.5 mm voxels holding the
+10-ellipsoid phantom scaled to +-100 mm; 1024 views over 360 degrees onto a
+1024x1024 detector of 0.4 mm pixels, SID 1000 mm / SDD 1500 mm; Poisson noise
+with I0 = 1e5 photons per ray, then the log transform.  One step = one outer
+iteration: a SART sweep over all 1024 views in the reference's view-by-view
+order (relaxation 1.0), TV-FGP with 20 inner iterations (lambda 1e-3) and the
+FISTA momentum, all device resident.  Inputs are in HBM before timing starts.
+
+value = fwd+back GRays/s of the FP32 path = rays per sweep / (time of the
+Siddon correction kernel + time of the voxel-driven update kernel), both timed
+live with HIP events on the context stream.  The same run reports the FP16
+path, seconds per iteration of both, the roofline of the dominant kernel and
+the CPU oracle (OpenMP restatement of the reference kernels) on a bounded
+sample of the same sweep.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "fwd+back GRays/s; s per SART-FISTA iter at 512³/1024 views; FP16 vs FP32"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(ctx, g, meas, dtype, steps, warmup, beta, lam, niter):
+    import torch
+    x = torch.zeros(g.vol_shape, device="cuda", dtype=dtype)
+    xp = torch.zeros_like(x)
+    t = 1.0
+    for _ in range(warmup):
+        t = ctx.sart_fista_iteration(g, x, xp, meas, t, beta=beta, subset=1, fgp_lambda=lam, fgp_niter=niter)
+    torch.cuda.synchronize()
+    ctx.profile(True, reset=True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        t = ctx.sart_fista_iteration(g, x, xp, meas, t, beta=beta, subset=1, fgp_lambda=lam, fgp_niter=niter)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    prof = ctx.profile_read()
+    ctx.profile(False, reset=False)
+    return ms / steps, prof, xp
+
+
+def cpu_baseline(g, x_host, meas_host, nviews):
+    """Oracle SART correction + voxel update for the first `nviews` views of the
+    same sweep, FP32, OpenMP over all host cores."""
+    import numpy as np
+    from oracle import oracle as orc
+    threads = orc.num_threads()
+    v = np.zeros(g.vol_shape, dtype=np.float32)
+    t0 = time.perf_counter()
+    for j in range(nviews):
+        c = orc.sart_correction(g, v, meas_host[j:j + 1], view0=j, nviews=1, prec=32)
+        v = orc.backproject_voxel(g, v, c, view0=j, nviews=1, beta=1.0, mode=1, prec=32)
+    dt = time.perf_counter() - t0
+    rays = nviews * g.nrows * g.ncols
+    return {"value": rays / dt / 1e9, "unit": "GRays/s", "cores": threads, "kind": "port",
+            "sample": f"first {nviews} views of the 512^3 / 1024-view SART sweep (correction + voxel update), "
+                      f"FP32, {dt:.1f} s, OpenMP {threads} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-views", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from gpu_amd.core import Context
+    from gpu_amd.geometry import repo_scale_512
+    from gpu_amd.phantom import phantom, poisson_log
+
+    g = repo_scale_512()
+    beta, lam, niter = 1.0, 1e-3, 20
+    ctx = Context(0)
+    log("setup: phantom + projections + noise")
+    x_true = phantom(g, seed=0, scale=2.5, device="cuda")
+    clean = ctx.forward(g, x_true)
+    meas32 = poisson_log(clean, i0=1e5, seed=1)
+    del clean
+    torch.cuda.synchronize()
+
+    log(f"FP32 path: warmup {args.warmup}, steps {args.steps}")
+    ms32, prof32, x32 = run_path(ctx, g, meas32, torch.float32, args.steps, args.warmup, beta, lam, niter)
+    fwd_ms, fwd_n = prof32["forward"]
+    bp_ms, bp_n = prof32["voxel_bp"]
+    rays_per_sweep = g.rays
+    grays = rays_per_sweep * args.steps / ((fwd_ms + bp_ms) * 1e-3) / 1e9
+    log(f"FP32: {ms32 / 1e3:.3f} s/iter, forward {fwd_ms / fwd_n:.3f} ms x {fwd_n}, "
+        f"voxel bp {bp_ms / bp_n:.3f} ms x {bp_n}, GRays/s {grays:.3f}")
+
+    V, P = g.nvox, g.nrows * g.ncols
+    kernels = {
+        "forward": (fwd_ms / fwd_n, 4 * V + 4 * P + 4 * P),     # read f, read b, write c
+        "voxel_bp": (bp_ms / bp_n, 4 * P + 2 * 4 * V),           # read c, read+write f
+    }
+    dom = max(kernels, key=lambda k: prof32[k][0])
+    avg_ms, abytes = kernels[dom]
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get(dom)
+
+    result = {
+        "metric": METRIC,
+        "value": round(grays, 4),
+        "unit": "GRays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 10-ellipsoid phantom, Poisson I0=1e5 + log",
+        "config": {
+            "workload": "SART-FISTA-TV iteration, 512^3 x 0.5 mm, 1024 views x 1024^2 x 0.4 mm, SID/SDD 1000/1500 mm",
+            "sart": "view by view (subset 1), relaxation 1.0",
+            "tv_fgp": {"inner_iterations": niter, "lambda": lam},
+            "rays_per_sweep": rays_per_sweep,
+        },
+        "s_per_iter": {"f32": round(ms32 / 1e3, 4)},
+        "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof32.items() if v[1]},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": abytes},
+    }
+
+    if not args.no_fp16:
+        log("FP16 path")
+        meas16 = meas32.to(torch.float16)
+        ms16, prof16, x16 = run_path(ctx, g, meas16, torch.float16, args.steps, args.warmup, beta, lam, niter)
+        f16 = prof16["forward"][0] + prof16["voxel_bp"][0]
+        grays16 = rays_per_sweep * args.steps / (f16 * 1e-3) / 1e9
+        diff = (x16.float() - x32).double()
+        rmse = float(torch.sqrt((diff * diff).mean()))
+        result["s_per_iter"]["f16"] = round(ms16 / 1e3, 4)
+        result["fp16"] = {"grays": round(grays16, 4), "speedup_vs_f32": round(ms32 / ms16, 3),
+                          "rmse_vs_f32_per_mm": rmse,
+                          "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof16.items() if v[1]}}
+        log(f"FP16: {ms16 / 1e3:.3f} s/iter, GRays/s {grays16:.3f}, speedup {ms32 / ms16:.3f}, rmse {rmse:.3e}")
+        del meas16
+
+    if not args.no_cpu:
+        log(f"CPU baseline: {args.cpu_views} views")
+        meas_host = meas32[: args.cpu_views].cpu().numpy()
+        result["cpu_baseline"] = cpu_baseline(g, None, meas_host, args.cpu_views)
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

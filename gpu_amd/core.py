@@ -25,6 +25,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libctcore.so"
 F32, F16, BF16, F64 = 0, 1, 2, 3
 _DT = {torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16, torch.float64: F64}
 ORDER = {"ascending": 0, "interleaved": 1}
+KERNELS = ("forward", "voxel_bp", "adjoint", "fgp", "fista", "fdk_bp", "ramp")
 WINDOW = {"ramp": 0, "hann": 1}
 
 
@@ -74,6 +75,9 @@ def _load() -> ctypes.CDLL:
         "ct_fdk": ([P, G, P, I, P, I, I], I),
         "ct_sart_fista_iteration": ([P, G, P, P, I, P, I, D, I, I, D, I, DP], I),
         "ct_convert": ([P, P, I, P, I, I64], I),
+        "ct_profile_enable": ([P, I], I),
+        "ct_profile_reset": ([P], I),
+        "ct_profile_read": ([P, I, DP, ctypes.POINTER(ctypes.c_int64)], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -148,6 +152,20 @@ class Context:
     @property
     def scratch_bytes(self) -> int:
         return int(_lib.ct_scratch_bytes(self._h))
+
+    # ---- kernel timing (HIP events on the context stream)
+    def profile(self, enable: bool = True, reset: bool = True) -> None:
+        if reset:
+            self._check(_lib.ct_profile_reset(self._h))
+        self._check(_lib.ct_profile_enable(self._h, int(enable)))
+
+    def profile_read(self) -> dict[str, tuple[float, int]]:
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            self._check(_lib.ct_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
 
     # ---- projector pair
     def forward(self, g: Geometry, vol: torch.Tensor, proj: torch.Tensor | None = None,

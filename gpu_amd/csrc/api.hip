@@ -89,6 +89,23 @@ const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fd
     return ctx->angles.ptr;
 }
 
+void prof_drain(ct_context *ctx) {
+    if (ctx->prof_recs.empty()) return;
+    hipEventSynchronize(ctx->prof_recs.back().b);
+    for (auto &r : ctx->prof_recs) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ctx->prof_ms[r.kid] += ms;
+            ctx->prof_n[r.kid] += 1;
+        } else {
+            hipGetLastError();
+        }
+        ctx->prof_pool.push_back(r.a);
+        ctx->prof_pool.push_back(r.b);
+    }
+    ctx->prof_recs.clear();
+}
+
 ct_status check_ptr(ct_context *ctx, const void *p, ct_dtype dt, const char *name) {
     if (!p) return fail(ctx, CT_ERR_NULL, std::string(name) + " is NULL");
     if (((uintptr_t)p) % dtype_size(dt)) return fail(ctx, CT_ERR_ALIGN, std::string(name) + " is not aligned to its element size");
@@ -224,9 +241,35 @@ ct_status ct_destroy(ct_context *ctx) {
     if (!ctx) return CT_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    prof_drain(ctx);
+    for (hipEvent_t e : ctx->prof_pool) hipEventDestroy(e);
     for (auto &b : ctx->slots) if (b.ptr) hipFree(b.ptr);
     if (ctx->angles.ptr) hipFree(ctx->angles.ptr);
     delete ctx;
+    return CT_OK;
+}
+
+ct_status ct_profile_enable(ct_context *ctx, int32_t enable) {
+    if (!ctx) return CT_ERR_NULL;
+    ctx->prof = enable != 0;
+    return CT_OK;
+}
+
+ct_status ct_profile_reset(ct_context *ctx) {
+    if (!ctx) return CT_ERR_NULL;
+    DeviceGuard dg(ctx);
+    prof_drain(ctx);
+    for (int k = 0; k < CT_K_COUNT; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; }
+    return CT_OK;
+}
+
+ct_status ct_profile_read(ct_context *ctx, ct_kernel_id kernel, double *total_ms, int64_t *launches) {
+    if (!ctx || !total_ms || !launches) return CT_ERR_NULL;
+    if ((int)kernel < 0 || (int)kernel >= CT_K_COUNT) return fail(ctx, CT_ERR_INVALID_ARG, "unknown kernel id");
+    DeviceGuard dg(ctx);
+    prof_drain(ctx);
+    *total_ms = ctx->prof_ms[kernel];
+    *launches = ctx->prof_n[kernel];
     return CT_OK;
 }
 

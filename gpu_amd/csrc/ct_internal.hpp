@@ -33,7 +33,12 @@ template <> struct St<__hip_bfloat16> {
     __device__ __forceinline__ static void st(__hip_bfloat16 *p, int64_t i, float v) { p[i] = __float2bfloat16(v); }
 };
 
-template <typename S> __device__ __forceinline__ float ldf(const S *p, int64_t i) { return St<S>::ld(p, i); }
+// Load in the arithmetic type of the storage: double for F64, float otherwise.
+template <typename S> struct LdT { using type = float; };
+template <> struct LdT<double> { using type = double; };
+template <typename S> __device__ __forceinline__ typename LdT<S>::type ldf(const S *p, int64_t i) {
+    return St<S>::ld(p, i);
+}
 
 inline size_t dtype_size(ct_dtype d) {
     switch (d) {
@@ -96,9 +101,48 @@ struct ct_context {
     ct::DevBuf angles;
     std::vector<double> angle_key;
     size_t scratch_total = 0;
+    // kernel timing (ct_profile_*)
+    bool prof = false;
+    struct ProfRec { int kid; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[CT_K_COUNT] = {};
+    int64_t prof_n[CT_K_COUNT] = {};
 };
 
 namespace ct {
+
+// Drains recorded event pairs into the per-kernel totals (synchronizes).
+void prof_drain(ct_context *ctx);
+
+// Brackets the launches of one profiled kernel call with an event pair.
+struct ProfScope {
+    ct_context *ctx;
+    int kid;
+    hipEvent_t a = nullptr;
+    ProfScope(ct_context *c, int k) : ctx(c), kid(k) {
+        if (!ctx->prof) return;
+        a = take();
+        hipEventRecord(a, ctx->stream);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = take();
+        hipEventRecord(b, ctx->stream);
+        ctx->prof_recs.push_back({kid, a, b});
+        if (ctx->prof_recs.size() >= 8192) prof_drain(ctx);
+    }
+    hipEvent_t take() {
+        if (!ctx->prof_pool.empty()) {
+            hipEvent_t e = ctx->prof_pool.back();
+            ctx->prof_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    }
+};
 
 enum Slot {
     SLOT_CORR = 0,     // SART correction images

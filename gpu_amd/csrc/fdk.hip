@@ -210,6 +210,7 @@ static ct_status run_ramp(ct_context *ctx, const ct_geometry &g, const void *in,
     const int64_t rows = (int64_t)nimg * g.nrows;
     const int64_t pairs = (rows + 1) / 2;
     if (pairs > 2147483647LL) return fail(ctx, CT_ERR_OVERFLOW, "ramp filter: too many rows");
+    ProfScope ps(ctx, CT_K_RAMP);
     if (pow2)
         hipLaunchKernelGGL((k_ramp_fft<SI, SO>), dim3((unsigned)pairs), dim3(256), lds, ctx->stream, (const SI *)in,
                            (SO *)out, R, rows, pad, logn, (const float2 *)dtw, (const float *)dH);
@@ -330,6 +331,7 @@ static ct_status run_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *f
     const T dth = d.us_rate * pi / (T)180;
     const double w = (double)dth * 360.0 / (double)(d.us_rate * (T)g.nviews);
     dim3 block(64, 4, 1), grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
+    ProfScope ps(ctx, CT_K_FDK_BP);
     hipLaunchKernelGGL((k_fdk_bp<T, SP, SV>), grid, block, 0, ctx->stream, d, ang, (const SP *)filt, (SV *)vol, view0,
                        nv, (T)w);
     return launch_check(ctx, "fdk backprojection");

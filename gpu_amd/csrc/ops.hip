@@ -294,6 +294,7 @@ __global__ void k_fista(S *x, S *xp, S *y, int64_t n, T gamma) {
 }
 ct_status launch_fista(ct_context *ctx, void *x, void *xp, void *y, int64_t n, ct_dtype dt, double gamma) {
     dim3 g = grid_for(n), b(256);
+    ProfScope ps(ctx, CT_K_FISTA);
     switch (dt) {
     case CT_F32: hipLaunchKernelGGL((k_fista<float, float>), g, b, 0, ctx->stream, (float *)x, (float *)xp, (float *)y, n, (float)gamma); break;
     case CT_F16: hipLaunchKernelGGL((k_fista<__half, float>), g, b, 0, ctx->stream, (__half *)x, (__half *)xp, (__half *)y, n, (float)gamma); break;
@@ -363,6 +364,7 @@ static ct_status run_fgp(ct_context *ctx, int M, int N, int Z, void *vol, double
     for (int it = 0; it < niter; ++it) {
         const T tk = ((T)1 + std::sqrt((T)1 + (T)4 * tk1 * tk1)) / (T)2;
         const T g = (tk1 - (T)1) / tk;
+        ProfScope ps(ctx, CT_K_FGP);
         hipLaunchKernelGGL((k_fgp_L<S, T>), gr, b, 0, ctx->stream, (const S *)vol, r, s, t, (S *)nullptr, L, M, N, Z, lam);
         hipLaunchKernelGGL((k_fgp_LT<T>), gr, b, 0, ctx->stream, r, s, t, (const T *)L, o, p, q, M, N, Z, g, a12);
         tk1 = tk;

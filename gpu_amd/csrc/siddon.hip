@@ -206,6 +206,7 @@ static ct_status run_forward(ct_context *ctx, const ct_geometry &g, const void *
     DGeo<T> d = derive<T>(g);
     const int nimg = nv * d.NS;
     dim3 block(256);
+    ProfScope ps(ctx, CT_K_FORWARD);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
@@ -279,6 +280,7 @@ static ct_status run_adjoint(ct_context *ctx, const ct_geometry &g, const void *
     DGeo<T> d = derive<T>(g);
     const int nimg = nv * d.NS;
     dim3 block(256);
+    ProfScope ps(ctx, CT_K_ADJOINT);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);

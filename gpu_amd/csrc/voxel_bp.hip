@@ -210,6 +210,7 @@ static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const 
     DGeo<T> d = derive<T>(g);
     dim3 block(64, 4, 1);
     dim3 grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
+    ProfScope ps(ctx, CT_K_VOXEL_BP);
     hipLaunchKernelGGL((k_voxel_bp<T, SV>), grid, block, 0, ctx->stream, d, ang, (SV *)vol, (const T *)corr, first,
                        stride, nv, (T)beta, mode);
     return launch_check(ctx, "voxel backprojection");

@@ -265,6 +265,26 @@ ct_status ct_sart_fista_iteration(ct_context *ctx, const ct_geometry *g,
                                   double fgp_lambda, int32_t fgp_niter,
                                   double *t_state);
 
+/* ---- kernel timing ------------------------------------------------------ */
+/* HIP-event timing of every launch of the hot kernels, on the context's
+ * stream (the reference brackets its host calls with cudaEvent timers,
+ * V63/host_IterativeRecon_CBCT.c:5-8,55-61).  While enabled, each launch of a
+ * profiled kernel is bracketed by an event pair; ct_profile_read synchronizes
+ * and returns the summed milliseconds and launch count of one kernel class. */
+typedef enum ct_kernel_id {
+    CT_K_FORWARD = 0,       /* Siddon ray sum / SART correction */
+    CT_K_VOXEL_BP = 1,      /* voxel-driven backprojection + update */
+    CT_K_ADJOINT = 2,       /* matched adjoint scatter */
+    CT_K_FGP = 3,           /* one TV-FGP inner iteration (both kernels) */
+    CT_K_FISTA = 4,         /* FISTA momentum */
+    CT_K_FDK_BP = 5,        /* FDK backprojection */
+    CT_K_RAMP = 6,          /* ramp filter */
+    CT_K_COUNT = 7
+} ct_kernel_id;
+ct_status ct_profile_enable(ct_context *ctx, int32_t enable);
+ct_status ct_profile_reset(ct_context *ctx);
+ct_status ct_profile_read(ct_context *ctx, ct_kernel_id kernel, double *total_ms, int64_t *launches);
+
 /* ---- utilities -------------------------------------------------------- */
 /* Element conversion between storage types (n elements, device). */
 ct_status ct_convert(ct_context *ctx, const void *src, ct_dtype src_dtype,
