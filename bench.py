@@ -87,6 +87,9 @@ def main() -> int:
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--pmc-sample", type=int, default=0,
+                    help="counter-collection mode: run N views of the sweep kernels at full shape plus a "
+                         "1 GiB device copy (calibration) and exit without the JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -107,6 +110,20 @@ def main() -> int:
     meas32 = poisson_log(clean, i0=1e5, seed=1)
     del clean
     torch.cuda.synchronize()
+
+    if args.pmc_sample:
+        # known-size copy: 2^28 floats read + written (1 GiB each way)
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        vol = torch.zeros(g.vol_shape, device="cuda")
+        for j in range(args.pmc_sample):
+            c = ctx.sart_correction(g, vol, meas32[j:j + 1], view0=j, nviews=1)
+            ctx.backproject_voxel(g, vol, c, view0=j, nviews=1, beta=beta, mode=1)
+        torch.cuda.synchronize()
+        log(f"pmc sample done: {args.pmc_sample} views")
+        return 0
 
     log(f"FP32 path: warmup {args.warmup}, steps {args.steps}")
     ms32, prof32, x32 = run_path(ctx, g, meas32, torch.float32, args.steps, args.warmup, beta, lam, niter)

@@ -75,6 +75,7 @@ def _load() -> ctypes.CDLL:
         "ct_fdk": ([P, G, P, I, P, I, I], I),
         "ct_sart_fista_iteration": ([P, G, P, P, I, P, I, D, I, I, D, I, DP], I),
         "ct_convert": ([P, P, I, P, I, I64], I),
+        "ct_set_math": ([P, I], I),
         "ct_profile_enable": ([P, I], I),
         "ct_profile_reset": ([P], I),
         "ct_profile_read": ([P, I, DP, ctypes.POINTER(ctypes.c_int64)], I),
@@ -152,6 +153,12 @@ class Context:
     @property
     def scratch_bytes(self) -> int:
         return int(_lib.ct_scratch_bytes(self._h))
+
+    def set_math(self, mode: str) -> None:
+        """'exact': the reference's FP32 operation sequence (bit-identical to the
+        CPU restatement); 'fast' (default): hardware reciprocal/sqrt and FMA in
+        the voxel-driven backprojector."""
+        self._check(_lib.ct_set_math(self._h, {"exact": 0, "fast": 1}[mode]))
 
     # ---- kernel timing (HIP events on the context stream)
     def profile(self, enable: bool = True, reset: bool = True) -> None:

@@ -249,6 +249,13 @@ ct_status ct_destroy(ct_context *ctx) {
     return CT_OK;
 }
 
+ct_status ct_set_math(ct_context *ctx, ct_math mode) {
+    if (!ctx) return CT_ERR_NULL;
+    if (mode != CT_MATH_EXACT && mode != CT_MATH_FAST) return fail(ctx, CT_ERR_INVALID_ARG, "unknown math mode");
+    ctx->math = mode;
+    return CT_OK;
+}
+
 ct_status ct_profile_enable(ct_context *ctx, int32_t enable) {
     if (!ctx) return CT_ERR_NULL;
     ctx->prof = enable != 0;

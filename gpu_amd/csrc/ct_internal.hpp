@@ -101,6 +101,7 @@ struct ct_context {
     ct::DevBuf angles;
     std::vector<double> angle_key;
     size_t scratch_total = 0;
+    ct_math math = CT_MATH_FAST;
     // kernel timing (ct_profile_*)
     bool prof = false;
     struct ProfRec { int kid; hipEvent_t a, b; };

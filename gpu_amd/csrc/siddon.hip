@@ -13,6 +13,7 @@
 // one branch-free step here: every axis whose next crossing equals the
 // minimum steps, which is the same set of cases without wave divergence.
 #include "ct_internal.hpp"
+#include <type_traits>
 
 namespace ct {
 
@@ -133,6 +134,150 @@ __device__ __forceinline__ T siddon_traverse(const DGeo<T> &d, T s, T c, int col
     return len;
 }
 
+// Ray setup shared by the batched forward sum: the reference's entry point,
+// plane indices and first crossings (:20-292), kept operation for operation.
+template <typename T> struct RayState {
+    T amax, ac, ax, ay, az, dist, x1x, x1y, x1z, ixd, iyd, izd;
+    int nx, ny, nz, vi, vj, vk, sgx, sgy, sgz;
+    bool hit;
+};
+
+template <typename T>
+__device__ __forceinline__ RayState<T> ray_setup(const DGeo<T> &d, T s, T c, int col, int row, int src) {
+    RayState<T> r;
+    const int M = d.M, N = d.N, Z = d.Z;
+    const T x2x = d.sx * c - d.sy * s;
+    const T x2y = d.sx * s + d.sy * c;
+    const T x2z = d.tb ? (T)(d.sz0 + (T)src * d.sdz) : d.sz0;
+    const T u = d.ymin + (T)col * d.du;
+    const T x1x = d.dod * c - u * s;
+    const T x1y = d.dod * s + u * c;
+    const T x1z = d.zmin + (T)row * d.dv;
+    const T ixd = (T)1 / (x2x - x1x);
+    const T iyd = (T)1 / (x2y - x1y);
+    const T izd = (T)1 / (x2z - x1z);
+    r.x1x = x1x; r.x1y = x1y; r.x1z = x1z; r.ixd = ixd; r.iyd = iyd; r.izd = izd;
+    T a0, a1;
+    a0 = (d.bx - x1x) * ixd;
+    a1 = (d.bx + d.vx * (T)M - x1x) * ixd;
+    const T axmin = dmin(a0, a1), axmax = dmax(a0, a1);
+    a0 = (d.by - x1y) * iyd;
+    a1 = (d.by + d.vy * (T)N - x1y) * iyd;
+    const T aymin = dmin(a0, a1), aymax = dmax(a0, a1);
+    a0 = (d.bz - x1z) * izd;
+    a1 = (d.bz + d.vz * (T)Z - x1z) * izd;
+    const T azmin = dmin(a0, a1), azmax = dmax(a0, a1);
+    const T amin = dmax(dmax(axmin, aymin), dmax(aymin, azmin));
+    const T amax = dmin(dmin(axmax, aymax), dmin(aymax, azmax));
+    r.amax = amax; r.ac = amin;
+    r.hit = !(amax <= amin);
+    if (!r.hit) return r;
+    int imin, imax, jmin, jmax, kmin, kmax;
+    const T ex = x2x - x1x, ey = x2y - x1y, ez = x2z - x1z;
+    if (x1x < x2x) {
+        imin = (amin == axmin) ? 1 : (int)dfloor((amin * ex + x1x - d.bx) * d.ivx) + 1;
+        imax = (amax == axmax) ? M : (int)dfloor((amax * ex + x1x - d.bx) * d.ivx);
+    } else {
+        imax = (amin == axmin) ? M - 1 : (int)dfloor((amin * ex + x1x - d.bx) * d.ivx);
+        imin = (amax == axmax) ? 0 : (int)dfloor((amax * ex + x1x - d.bx) * d.ivx) + 1;
+    }
+    if (x1y < x2y) {
+        jmin = (amin == aymin) ? 1 : (int)dfloor((amin * ey + x1y - d.by) * d.ivy) + 1;
+        jmax = (amax == aymax) ? N : (int)dfloor((amax * ey + x1y - d.by) * d.ivy);
+    } else {
+        jmax = (amin == aymin) ? N - 1 : (int)dfloor((amin * ey + x1y - d.by) * d.ivy);
+        jmin = (amax == aymax) ? 0 : (int)dfloor((amax * ey + x1y - d.by) * d.ivy) + 1;
+    }
+    const bool zpar = dabs(x1z - x2z) < d.vz * (T)1e-6;
+    if (zpar) {
+        kmin = (int)dfloor((x1z - d.bz) * d.ivz) + 1;
+        kmax = (int)dfloor((x1z - d.bz) * d.ivz);
+    } else if (x1z < x2z) {
+        kmin = (amin == azmin) ? 1 : (int)dfloor((amin * ez + x1z - d.bz) * d.ivz) + 1;
+        kmax = (amax == azmax) ? Z : (int)dfloor((amax * ez + x1z - d.bz) * d.ivz);
+    } else {
+        kmax = (amin == azmin) ? Z - 1 : (int)dfloor((amin * ez + x1z - d.bz) * d.ivz);
+        kmin = (amax == azmax) ? 0 : (int)dfloor((amax * ez + x1z - d.bz) * d.ivz) + 1;
+    }
+    r.sgx = (x1x < x2x) ? 1 : ((x1x > x2x) ? -1 : 0);
+    r.sgy = (x1y < x2y) ? 1 : ((x1y > x2y) ? -1 : 0);
+    r.sgz = (x1z < x2z) ? 1 : ((x1z > x2z) ? -1 : 0);
+    if (dabs(x1x - x2x) < d.vx * (T)1e-6) { r.ax = dinf<T>(); r.nx = imin; }
+    else { r.nx = (r.sgx > 0) ? imin : imax; r.ax = (d.vx * (T)r.nx + d.bx - x1x) * ixd; }
+    if (dabs(x1y - x2y) < d.vy * (T)1e-6) { r.ay = dinf<T>(); r.ny = jmin; }
+    else { r.ny = (r.sgy > 0) ? jmin : jmax; r.ay = (d.vy * (T)r.ny + d.by - x1y) * iyd; }
+    if (zpar) { r.az = dinf<T>(); r.nz = kmin; }
+    else { r.nz = (r.sgz > 0) ? kmin : kmax; r.az = (d.vz * (T)r.nz + d.bz - x1z) * izd; }
+    r.vi = (x1x < x2x) ? imin - 1 : imax;
+    r.vj = (x1y < x2y) ? jmin - 1 : jmax;
+    r.vk = (x1z < x2z) ? kmin - 1 : kmax;
+    r.dist = dsqrt(ex * ex + ey * ey + ez * ez);
+    return r;
+}
+
+// Ray sum with B steps generated per batch and their B gathers issued
+// together: the steps, segment lengths and the order of the two sums are
+// those of siddon_traverse, so with COMP = false the result is identical; only
+// the loads are in flight together instead of one at a time.  COMP = true
+// (fast mode) adds Kahan compensation to the ray sum and length, which keeps
+// long FP32 rays at FP64-like accuracy.
+template <typename T, typename SV, int B, bool COMP>
+__device__ __forceinline__ T siddon_sum(const DGeo<T> &d, T s, T c, int col, int row, int src,
+                                        const SV *__restrict__ f, T &len_out) {
+    RayState<T> r = ray_setup(d, s, c, col, row, src);
+    T sum = 0, len = 0, csum = 0, clen = 0;
+    if (r.hit) {
+        const unsigned M = d.M, N = d.N, Z = d.Z, MN = M * N;
+        int guard = d.M + d.N + d.Z + 4;
+        bool alive = true;
+        while (alive) {
+            unsigned idx[B];
+            T seg[B];
+            bool act[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const bool ok = alive && (r.amax - r.ac > loop_eps<T>()) && (guard > 0) &&
+                                (unsigned)r.vi < M && (unsigned)r.vj < N && (unsigned)r.vk < Z;
+                const T m = dmin(dmin(r.ax, r.ay), r.az);
+                act[u] = ok;
+                seg[u] = r.dist * (m - r.ac);
+                idx[u] = ok ? (unsigned)r.vk * MN + (unsigned)r.vj * M + (unsigned)r.vi : 0u;
+                if (ok) {
+                    --guard;
+                    r.ac = m;
+                    if (r.ax == m) { r.vi += r.sgx; r.nx += r.sgx; r.ax = (d.vx * (T)r.nx + d.bx - r.x1x) * r.ixd; }
+                    if (r.ay == m) { r.vj += r.sgy; r.ny += r.sgy; r.ay = (d.vy * (T)r.ny + d.by - r.x1y) * r.iyd; }
+                    if (r.az == m) { r.vk += r.sgz; r.nz += r.sgz; r.az = (d.vz * (T)r.nz + d.bz - r.x1z) * r.izd; }
+                }
+                alive = ok;
+            }
+            T fv[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) fv[u] = (T)ldf(f, idx[u]);
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (act[u]) {
+                    if constexpr (COMP) {
+                        const T yl = seg[u] - clen;
+                        const T tl = len + yl;
+                        clen = (tl - len) - yl;
+                        len = tl;
+                        const T ys = seg[u] * fv[u] - csum;
+                        const T ts = sum + ys;
+                        csum = (ts - sum) - ys;
+                        sum = ts;
+                    } else {
+                        len += seg[u];
+                        sum += seg[u] * fv[u];
+                    }
+                }
+            }
+        }
+    }
+    len_out = len;
+    return sum;
+}
+
 __device__ __forceinline__ void tile_coords(int &col, int &row) {
     const int tid = threadIdx.x;
     const int w = tid >> 6, l = tid & 63;
@@ -143,7 +288,7 @@ __device__ __forceinline__ void tile_coords(int &col, int &row) {
 // CMD 0: ray sum; CMD 1: SART correction (b - sum)/len
 // (V63/kernel_IterativeRecon_universal.cu:554-563).  blockIdx.z = q*NS + src
 // for batch slot q; view = first + q*stride; meas is indexed by view - mv0.
-template <typename T, typename SV, typename SO, typename SM, int CMD>
+template <typename T, typename SV, typename SO, typename SM, int CMD, bool COMP>
 __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict__ ang,
                                                  const SV *__restrict__ f, SO *__restrict__ out,
                                                  const SM *__restrict__ meas, int first, int stride,
@@ -155,8 +300,8 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
     const int q = img / d.NS, src = img - q * d.NS;
     const int v = first + q * stride;
     const T s = ang[2 * v], c = ang[2 * v + 1];
-    T sum = 0;
-    const T len = siddon_traverse(d, s, c, col, row, src, [&](int idx, T seg) { sum += seg * (T)ldf(f, idx); });
+    T len;
+    const T sum = siddon_sum<T, SV, 4, COMP>(d, s, c, col, row, src, f, len);
     const int64_t P = (int64_t)d.R * d.ZP;
     const int64_t pix = (int64_t)row * d.R + col;
     T r;
@@ -193,7 +338,7 @@ __global__ __launch_bounds__(256) void k_adjoint(DGeo<T> d, const T *__restrict_
     const T s = ang[2 * v], c = ang[2 * v + 1];
     const T len = siddon_traverse(d, s, c, col, row, src, [](int, T) {});
     if (len < d.vz * (T)1e-6) return;
-    siddon_traverse(d, s, c, col, row, src, [&](int idx, T seg) { atomic_add<A>(acc + idx, (A)(seg * y)); });
+    siddon_traverse(d, s, c, col, row, src, [&](int idx, T seg) { atomic_add<A>(acc + idx, (A)seg * (A)y); });
 }
 
 // ---------------------------------------------------------------- launchers
@@ -206,12 +351,19 @@ static ct_status run_forward(ct_context *ctx, const ct_geometry &g, const void *
     DGeo<T> d = derive<T>(g);
     const int nimg = nv * d.NS;
     dim3 block(256);
+    // compensated FP32 ray sums in fast mode; the exact mode keeps the
+    // reference's plain accumulation
+    const bool comp = std::is_same<T, float>::value && ctx->math == CT_MATH_FAST;
     ProfScope ps(ctx, CT_K_FORWARD);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
-        hipLaunchKernelGGL((k_forward<T, SV, SO, SM, CMD>), grid, block, 0, ctx->stream, d, ang, (const SV *)vol,
-                           (SO *)out, (const SM *)meas, first, stride, mv0, z0);
+        if (comp)
+            hipLaunchKernelGGL((k_forward<T, SV, SO, SM, CMD, true>), grid, block, 0, ctx->stream, d, ang,
+                               (const SV *)vol, (SO *)out, (const SM *)meas, first, stride, mv0, z0);
+        else
+            hipLaunchKernelGGL((k_forward<T, SV, SO, SM, CMD, false>), grid, block, 0, ctx->stream, d, ang,
+                               (const SV *)vol, (SO *)out, (const SM *)meas, first, stride, mv0, z0);
     }
     return launch_check(ctx, "forward");
 }
@@ -219,7 +371,7 @@ static ct_status run_forward(ct_context *ctx, const ct_geometry &g, const void *
 template <typename T, typename SV, typename SO, int CMD>
 static ct_status fwd_meas(ct_context *ctx, const ct_geometry &g, const void *vol, void *out, const void *meas,
                           ct_dtype mdt, int first, int stride, int nv, int mv0) {
-    if (CMD == 0) return run_forward<T, SV, SO, SO, CMD>(ctx, g, vol, out, meas, first, stride, nv, mv0);
+    if constexpr (CMD == 0) return run_forward<T, SV, SO, SO, CMD>(ctx, g, vol, out, meas, first, stride, nv, mv0);
     switch (mdt) {
     case CT_F32: return run_forward<T, SV, SO, float, CMD>(ctx, g, vol, out, meas, first, stride, nv, mv0);
     case CT_F16: return run_forward<T, SV, SO, __half, CMD>(ctx, g, vol, out, meas, first, stride, nv, mv0);
@@ -299,25 +451,22 @@ ct_status launch_adjoint(ct_context *ctx, const ct_geometry &g, const void *proj
         return run_adjoint<double, double, double>(ctx, g, proj, (double *)vol, view0, nv);
     }
     if (pdt == CT_F64) return fail(ctx, CT_ERR_DTYPE, "F64 projections need an F64 volume");
-    float *acc = (float *)vol;
+    // FP32 segments scattered into an FP64 accumulator (f64 atomics): the
+    // ~10^2 contributions per voxel are summed without FP32 rounding, then
+    // rounded once to the output type
     ct_status st;
-    if (vdt != CT_F32) {
-        acc = (float *)slot(ctx, SLOT_TMP_VOL, sizeof(float) * V, &st);
-        if (!acc) return st;
-    }
-    st = launch_fill(ctx, acc, CT_F32, V, 0.0);
+    double *acc = (double *)slot(ctx, SLOT_TMP_VOL, sizeof(double) * V, &st);
+    if (!acc) return st;
+    st = launch_fill(ctx, acc, CT_F64, V, 0.0);
     if (st != CT_OK) return st;
     switch (pdt) {
-    case CT_F32: st = run_adjoint<float, float, float>(ctx, g, proj, acc, view0, nv); break;
-    case CT_F16: st = run_adjoint<float, __half, float>(ctx, g, proj, acc, view0, nv); break;
-    case CT_BF16: st = run_adjoint<float, __hip_bfloat16, float>(ctx, g, proj, acc, view0, nv); break;
+    case CT_F32: st = run_adjoint<float, float, double>(ctx, g, proj, acc, view0, nv); break;
+    case CT_F16: st = run_adjoint<float, __half, double>(ctx, g, proj, acc, view0, nv); break;
+    case CT_BF16: st = run_adjoint<float, __hip_bfloat16, double>(ctx, g, proj, acc, view0, nv); break;
     default: return fail(ctx, CT_ERR_DTYPE, "unsupported projection dtype");
     }
     if (st != CT_OK) return st;
-    if (vdt != CT_F32) {
-        return launch_convert(ctx, acc, CT_F32, vol, vdt, V);
-    }
-    return CT_OK;
+    return launch_convert(ctx, acc, CT_F64, vol, vdt, V);
 }
 
 } // namespace ct

@@ -15,6 +15,7 @@
 // reference's row-major order, so the result is bit-identical to the
 // per-pixel form.
 #include "ct_internal.hpp"
+#include <type_traits>
 
 namespace ct {
 
@@ -147,7 +148,119 @@ __device__ __forceinline__ void voxel_wl(const DGeo<T> &d, T cx, T cy, T cz, T s
     }
 }
 
-template <typename T, typename SV>
+// FP32 fast form of voxel_wl: the same chord per (voxel, pixel) with the
+// hardware reciprocal in place of IEEE division, the hardware square root and
+// a branch-free pixel accumulation.  Every coordinate (pixel position, slab
+// plane, source offset) keeps the reference's multiply/add sequence: a ray
+// that lies exactly in a voxel boundary plane (zero divisor) then resolves
+// exactly as in the reference (x/0 = inf, 0/0 = NaN), and the remaining
+// ~1 ulp differences only move chords shorter than the 1e-6*vx cut.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, float cy, float cz, float s, float c,
+                                              float szs, const float *__restrict__ corr, float &W, float &L) {
+    if (d.tb) {
+        const float tc = cx * c + cy * s;
+        const float rk = (d.dso - d.dod) * frcp(tc - d.dod);
+        const float upper = (cz + d.vz - d.zmin) * rk + d.zmin;
+        const float lower = (cz - d.vz - d.zmax) * rk + d.zmax;
+        if (szs < lower || szs > upper) return;
+    }
+    const float hvx = 0.5f * d.vx, hvy = 0.5f * d.vy, hvz = 0.5f * d.vz;
+    const float xl = cx - hvx, xh = cx + hvx, yl = cy - hvy, yh = cy + hvy, zl = cz - hvz, zh = cz + hvz;
+    float minY = __builtin_inff(), maxY = -__builtin_inff(), minZ = __builtin_inff(), maxZ = -__builtin_inff();
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii) {
+            const float px = ii ? xh : xl, py = jj ? yh : yl;
+            const float t = px * c + py * s;
+            const float sv = -px * s + py * c;
+            const float ratio = (t - d.dod) * frcp(d.dso - t);
+            const float u = fmaf(ratio, sv - d.sy, sv);
+            const float w0 = fmaf(ratio, zl - szs, zl);
+            const float w1 = fmaf(ratio, zh - szs, zh);
+            minY = fminf(minY, u); maxY = fmaxf(maxY, u);
+            minZ = fminf(minZ, fminf(w0, w1)); maxZ = fmaxf(maxZ, fmaxf(w0, w1));
+        }
+    int y0 = (int)floorf(fmaf(minY - d.ymin, d.idu, 0.5f));
+    int y1 = (int)floorf(fmaf(maxY - d.ymin, d.idu, 0.5f));
+    int z0 = (int)floorf(fmaf(minZ - d.zmin, d.idv, 0.5f));
+    int z1 = (int)floorf(fmaf(maxZ - d.zmin, d.idv, 0.5f));
+    if ((y0 < 0 && y1 < 0) || (y0 > d.R - 1 && y1 > d.R - 1) || (z0 < 0 && z1 < 0) ||
+        (z0 > d.ZP - 1 && z1 > d.ZP - 1))
+        return;
+    y0 = max(y0, 0); y1 = min(y1, d.R - 1);
+    z0 = max(z0, 0); z1 = min(z1, d.ZP - 1);
+    const float sxw = d.sx * c - d.sy * s;
+    const float syw = d.sx * s + d.sy * c;
+    const float thr = d.vx * 1e-6f;
+    const int ncol = y1 - y0 + 1;
+    if (ncol > MAXC) {
+        // wide bounding boxes: per-pixel form
+        for (int j = z0; j <= z1; ++j) {
+            const float qz = d.zmin + (float)j * d.dv;
+            const float ez = szs - qz;
+            const float rz = frcp(ez);
+            const float az0 = (zl - qz) * rz, az1 = (zh - qz) * rz;
+            const float loz = fminf(az0, az1), hiz = fmaxf(az0, az1);
+            for (int i = y0; i <= y1; ++i) {
+                const float u = d.ymin + (float)i * d.du;
+                const float qx = d.dod * c - u * s, qy = d.dod * s + u * c;
+                const float ex = sxw - qx, ey = syw - qy;
+                const float rx = frcp(ex), ry = frcp(ey);
+                const float ax0 = (xl - qx) * rx, ax1 = (xh - qx) * rx;
+                const float ay0 = (yl - qy) * ry, ay1 = (yh - qy) * ry;
+                const float amin = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), loz);
+                const float amax = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), hiz);
+                const float len = fsqrt(fmaf(ex, ex, fmaf(ey, ey, ez * ez))) * (amax - amin);
+                if (amax > amin && len > thr) {
+                    W = fmaf(len, corr[(int64_t)j * d.R + i], W);
+                    L += len;
+                }
+            }
+        }
+        return;
+    }
+    float lo[MAXC], hi[MAXC], a2[MAXC];
+#pragma unroll
+    for (int cc = 0; cc < MAXC; ++cc) {
+        if (cc < ncol) {
+            const float u = d.ymin + (float)(y0 + cc) * d.du;
+            const float qx = d.dod * c - u * s, qy = d.dod * s + u * c;
+            const float ex = sxw - qx, ey = syw - qy;
+            const float rx = frcp(ex), ry = frcp(ey);
+            const float ax0 = (xl - qx) * rx, ax1 = (xh - qx) * rx;
+            const float ay0 = (yl - qy) * ry, ay1 = (yh - qy) * ry;
+            lo[cc] = fmaxf(fminf(ax0, ax1), fminf(ay0, ay1));
+            hi[cc] = fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1));
+            a2[cc] = fmaf(ex, ex, ey * ey);
+        }
+    }
+    for (int j = z0; j <= z1; ++j) {
+        const float qz = d.zmin + (float)j * d.dv;
+        const float ez = szs - qz;
+        const float rz = frcp(ez);
+        const float az0 = (zl - qz) * rz, az1 = (zh - qz) * rz;
+        const float loz = fminf(az0, az1), hiz = fmaxf(az0, az1);
+        const float c2 = ez * ez;
+        const float *crow = corr + (int64_t)j * d.R + y0;
+#pragma unroll
+        for (int cc = 0; cc < MAXC; ++cc) {
+            if (cc < ncol) {
+                const float span = fminf(hi[cc], hiz) - fmaxf(lo[cc], loz);
+                const float len = fsqrt(a2[cc] + c2) * span;
+                const bool ok = (span > 0.0f) & (len > thr);
+                const float cv = crow[cc];
+                W = ok ? fmaf(len, cv, W) : W;
+                L = ok ? L + len : L;
+            }
+        }
+    }
+}
+
+template <typename T, typename SV, bool FAST>
 __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict__ ang, SV *__restrict__ f,
                                                   const T *__restrict__ corr, int first, int stride, int nv,
                                                   T beta, int mode) {
@@ -179,7 +292,15 @@ __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict
             for (int src = 0; src < d.NS; ++src) {
                 const T szs = d.tb ? (T)(d.sz0 + (T)src * d.sdz) : d.sz0;
                 const T *cimg = corr + ((int64_t)q * d.NS + src) * P;
-                if (d.tb) {
+                if constexpr (FAST) {
+                    if (d.tb) {
+                        float w = 0, l = 0;
+                        voxel_wl_fast(d, cx, cy, cz, s, c, szs, cimg, w, l);
+                        W += w; L += l;
+                    } else {
+                        voxel_wl_fast(d, cx, cy, cz, s, c, szs, cimg, W, L);
+                    }
+                } else if (d.tb) {
                     T w = 0, l = 0;
                     voxel_wl(d, cx, cy, cz, s, c, szs, cimg, w, l);
                     W += w; L += l;
@@ -211,8 +332,15 @@ static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const 
     dim3 block(64, 4, 1);
     dim3 grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
     ProfScope ps(ctx, CT_K_VOXEL_BP);
-    hipLaunchKernelGGL((k_voxel_bp<T, SV>), grid, block, 0, ctx->stream, d, ang, (SV *)vol, (const T *)corr, first,
-                       stride, nv, (T)beta, mode);
+    if constexpr (std::is_same<T, float>::value) {
+        if (ctx->math == CT_MATH_FAST) {
+            hipLaunchKernelGGL((k_voxel_bp<T, SV, true>), grid, block, 0, ctx->stream, d, ang, (SV *)vol,
+                               (const T *)corr, first, stride, nv, (T)beta, mode);
+            return launch_check(ctx, "voxel backprojection");
+        }
+    }
+    hipLaunchKernelGGL((k_voxel_bp<T, SV, false>), grid, block, 0, ctx->stream, d, ang, (SV *)vol, (const T *)corr,
+                       first, stride, nv, (T)beta, mode);
     return launch_check(ctx, "voxel backprojection");
 }
 

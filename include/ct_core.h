@@ -265,6 +265,19 @@ ct_status ct_sart_fista_iteration(ct_context *ctx, const ct_geometry *g,
                                   double fgp_lambda, int32_t fgp_niter,
                                   double *t_state);
 
+/* ---- arithmetic mode ---------------------------------------------------- */
+/* CT_MATH_EXACT keeps the reference's FP32 operation sequence (IEEE division
+ * and square root, no contraction): results are bit-identical to the CPU
+ * restatement.  CT_MATH_FAST (the default) lets the FP32 voxel-driven
+ * backprojector use the hardware reciprocal / square root and fused
+ * multiply-adds; it stays within the parity tolerances (1e-5 relative L2 per
+ * update, 1e-4 on reconstructions).  FP64 paths are always exact. */
+typedef enum ct_math {
+    CT_MATH_EXACT = 0,
+    CT_MATH_FAST = 1
+} ct_math;
+ct_status ct_set_math(ct_context *ctx, ct_math mode);
+
 /* ---- kernel timing ------------------------------------------------------ */
 /* HIP-event timing of every launch of the hot kernels, on the context's
  * stream (the reference brackets its host calls with cudaEvent timers,

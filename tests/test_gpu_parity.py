@@ -61,6 +61,44 @@ def test_forward_fp32(ctx, name):
     assert r <= 1e-5
 
 
+@pytest.mark.parametrize("name", list(GEOMS))
+def test_exact_mode_is_bit_identical(ctx, name):
+    """CT_MATH_EXACT reproduces the reference's FP32 operation sequence: the
+    forward projection and two SART sweeps equal the CPU restatement bit for bit."""
+    g = GEOMS[name]
+    x = phantom(g, seed=21, scale=0.3).numpy()
+    ctx.set_math("exact")
+    try:
+        got = host(ctx.forward(g, dev(x)))
+        b = orc.forward(g, x, prec=32)
+        v = torch.zeros(g.vol_shape, device="cuda")
+        bd = dev(b)
+        ref = np.zeros_like(x)
+        for _ in range(2):
+            ctx.sart_sweep(g, v, bd, beta=0.9, subset=2, order="interleaved")
+            ref = orc.sart_sweep(g, ref, b, beta=0.9, subset=2, order="interleaved", prec=32)
+        rec = host(v)
+    finally:
+        ctx.set_math("fast")
+    assert np.array_equal(got, b.astype(np.float64))
+    assert np.array_equal(rec, ref.astype(np.float64))
+
+
+def test_fast_forward_accuracy_vs_fp64(ctx):
+    """Fast mode (Kahan-compensated ray sums) at 128^3 / 256^2: error against
+    the FP64 oracle no worse than the plain FP32 restatement's.  The residual
+    ~1e-6 comes from FP32 crossing positions (alpha spans the whole
+    source-detector line), which compensation does not change."""
+    g = plain_sart_128().with_views(4)
+    x = phantom(g, seed=0).numpy()
+    p64 = orc.forward(g, x.astype(np.float64), prec=64)
+    p32 = orc.forward(g, x, prec=32)
+    got = host(ctx.forward(g, dev(x)))
+    e_gpu, e_plain = rel(got, p64), rel(p32, p64)
+    print(f"fast vs FP64 {e_gpu:.3e}, plain FP32 vs FP64 {e_plain:.3e}")
+    assert e_gpu <= 1.05 * e_plain and e_gpu <= 2e-6
+
+
 @pytest.mark.parametrize("name", ["small", "aniso", "multisource"])
 def test_forward_fp64(ctx, name):
     g = GEOMS[name]
@@ -91,20 +129,30 @@ def test_sart_correction(ctx, name):
     assert rel(got, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("math_mode", ["exact", "fast"])
 @pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource"])
 @pytest.mark.parametrize("mode,nv", [(1, 1), (0, 1), (1, 4)])
-def test_backproject_voxel(ctx, name, mode, nv):
+def test_backproject_voxel(ctx, name, mode, nv, math_mode):
+    """exact: the reference's FP32 operation sequence, bit-identical to the
+    oracle; fast: hardware reciprocal/sqrt, held to the 1e-5 bar."""
     g = GEOMS[name]
     rng = np.random.default_rng(5)
     x = phantom(g, seed=14, scale=0.3).numpy()
     corr = rng.standard_normal((nv, *g.proj_shape[1:])).astype(np.float32) * 0.01
     ref = orc.backproject_voxel(g, x, corr, view0=2, nviews=nv, beta=0.7, mode=mode, prec=32)
     v = dev(x)
-    ctx.backproject_voxel(g, v, dev(corr), view0=2, nviews=nv, beta=0.7, mode=mode)
-    got = host(v)
+    ctx.set_math(math_mode)
+    try:
+        ctx.backproject_voxel(g, v, dev(corr), view0=2, nviews=nv, beta=0.7, mode=mode)
+        got = host(v)
+    finally:
+        ctx.set_math("fast")
     r = rel(got, ref)
-    print(f"{name} mode {mode} nv {nv}: rel {r:.3e} max {np.abs(got - ref).max():.3e}")
-    assert r <= 1e-5
+    print(f"{name} mode {mode} nv {nv} {math_mode}: rel {r:.3e} max {np.abs(got - ref).max():.3e}")
+    if math_mode == "exact":
+        assert r == 0.0
+    else:
+        assert r <= 1e-5
 
 
 def test_backproject_voxel_fp64(ctx):
