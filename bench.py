@@ -7,7 +7,8 @@ Workload (synthetic, seeded): 512^3 volume of 0.5 mm voxels holding the
 1024x1024 detector of 0.4 mm pixels, SID 1000 mm / SDD 1500 mm; Poisson noise
 with I0 = 1e5 photons per ray, then the log transform.  One step = one outer
 iteration: a SART sweep over all 1024 views in the reference's view-by-view
-order (relaxation 1.0), TV-FGP with 20 inner iterations (lambda 1e-3) and the
+order (relaxation 2/32, the reference's 2/Lip_con), TV-FGP with 20 inner
+iterations (lambda 1e-3) and the
 FISTA momentum, all device resident.  Inputs are in HBM before timing starts.
 
 value = fwd+back GRays/s of the FP32 path = rays per sweep / (time of the
@@ -61,6 +62,47 @@ def run_path(ctx, g, meas, dtype, steps, warmup, beta, lam, niter):
     return ms / steps, prof, xp
 
 
+def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
+    """Per-iteration RMSE / relative L2 of the FP32, FP16 and BF16 paths against
+    the FP64 path (same data, same schedule) and against the phantom."""
+    import torch
+    paths = {"f64": torch.float64, "f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}
+    state = {}
+    for k, dt in paths.items():
+        meas = meas32 if dt in (torch.float32, torch.float64) else meas32.to(dt)
+        state[k] = [torch.zeros(g.vol_shape, device="cuda", dtype=dt), torch.zeros(g.vol_shape, device="cuda", dtype=dt),
+                    1.0, meas, 0.0]
+    out = {"workload": f"512^3 / 1024 views SART(subset 1, beta {beta}) + TV-FGP({niter}, {lam}) + FISTA",
+           "reference": "f64 path on the GPU (FP64 instantiation of the same kernels)", "iterations": []}
+    truth = x_true.double()
+    tn = float(torch.linalg.vector_norm(truth))
+    for it in range(iters):
+        rec = {"iteration": it + 1}
+        for k in paths:
+            x, xp, t, meas, _ = state[k]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            t = ctx.sart_fista_iteration(g, x, xp, meas, t, beta=beta, subset=1, fgp_lambda=lam, fgp_niter=niter)
+            e1.record()
+            torch.cuda.synchronize()
+            state[k][2] = t
+            state[k][4] += e0.elapsed_time(e1) / 1e3
+            log(f"accuracy: iteration {it + 1} {k} {e0.elapsed_time(e1) / 1e3:.2f} s")
+        ref = state["f64"][1]          # x_k lives in x_prev after the momentum step
+        rn = float(torch.linalg.vector_norm(ref))
+        for k in paths:
+            xk = state[k][1].double()
+            d = xk - ref
+            rec[k] = {"rmse_vs_f64": float(torch.sqrt((d * d).mean())),
+                      "rel_l2_vs_f64": float(torch.linalg.vector_norm(d)) / rn,
+                      "rmse_vs_phantom": float(torch.sqrt(((xk - truth) ** 2).mean())),
+                      "rel_l2_vs_phantom": float(torch.linalg.vector_norm(xk - truth)) / tn}
+        out["iterations"].append(rec)
+    out["seconds_per_iteration"] = {k: state[k][4] / iters for k in paths}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline(g, x_host, meas_host, nviews):
     """Oracle SART correction + voxel update for the first `nviews` views of the
     same sweep, FP32, OpenMP over all host cores."""
@@ -87,6 +129,10 @@ def main() -> int:
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--accuracy", type=int, default=0,
+                    help="accuracy mode: N SART-FISTA-TV iterations in FP64 (the GPU FP64 instantiation as the "
+                         "oracle), FP32, FP16 and BF16; prints per-iteration RMSE / relative L2 against FP64 and "
+                         "the phantom as one JSON line, then exits")
     ap.add_argument("--pmc-sample", type=int, default=0,
                     help="counter-collection mode: run N views of the sweep kernels at full shape plus a "
                          "1 GiB device copy (calibration) and exit without the JSON line")
@@ -102,8 +148,30 @@ def main() -> int:
     from gpu_amd.phantom import phantom, poisson_log
 
     g = repo_scale_512()
-    beta, lam, niter = 1.0, 1e-3, 20
+    # relaxation 2/Lip_con = 2/32, the reference's SART-FISTA step
+    # (V63/main_recon.cu:58,161); FGP argument lambda = 1e-3
+    beta, lam, niter = 0.0625, 1e-3, 20
     ctx = Context(0)
+    if args.pmc_sample:
+        # counter-collection mode: a cheap input (one ellipsoid-free ball) so the
+        # profiled run has few dispatches besides the kernels of interest
+        zz, yy, xx = torch.meshgrid(*(torch.arange(n, device="cuda", dtype=torch.float32) - n / 2 + 0.5
+                                      for n in g.vol_shape), indexing="ij")
+        ball = ((xx * xx + yy * yy + zz * zz) < (0.4 * g.nx) ** 2).to(torch.float32) * 0.02
+        meas = ctx.forward(g, ball, view0=0, nviews=args.pmc_sample)
+        # calibration: the library's own element copy (k_convert F32 -> F32) of
+        # 2^28 floats, 1 GiB read + 1 GiB written
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        ctx.convert(src, dst)
+        torch.cuda.synchronize()
+        vol = torch.zeros(g.vol_shape, device="cuda")
+        for j in range(args.pmc_sample):
+            c = ctx.sart_correction(g, vol, meas[j:j + 1], view0=j, nviews=1)
+            ctx.backproject_voxel(g, vol, c, view0=j, nviews=1, beta=beta, mode=1)
+        torch.cuda.synchronize()
+        log(f"pmc sample done: {args.pmc_sample} views")
+        return 0
     log("setup: phantom + projections + noise")
     x_true = phantom(g, seed=0, scale=2.5, device="cuda")
     clean = ctx.forward(g, x_true)
@@ -111,19 +179,8 @@ def main() -> int:
     del clean
     torch.cuda.synchronize()
 
-    if args.pmc_sample:
-        # known-size copy: 2^28 floats read + written (1 GiB each way)
-        src = torch.ones(1 << 28, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        vol = torch.zeros(g.vol_shape, device="cuda")
-        for j in range(args.pmc_sample):
-            c = ctx.sart_correction(g, vol, meas32[j:j + 1], view0=j, nviews=1)
-            ctx.backproject_voxel(g, vol, c, view0=j, nviews=1, beta=beta, mode=1)
-        torch.cuda.synchronize()
-        log(f"pmc sample done: {args.pmc_sample} views")
-        return 0
+    if args.accuracy:
+        return accuracy(ctx, g, x_true, meas32, args.accuracy, beta, lam, niter)
 
     log(f"FP32 path: warmup {args.warmup}, steps {args.steps}")
     ms32, prof32, x32 = run_path(ctx, g, meas32, torch.float32, args.steps, args.warmup, beta, lam, niter)
@@ -162,7 +219,7 @@ def main() -> int:
         "data": "synthetic: seeded 10-ellipsoid phantom, Poisson I0=1e5 + log",
         "config": {
             "workload": "SART-FISTA-TV iteration, 512^3 x 0.5 mm, 1024 views x 1024^2 x 0.4 mm, SID/SDD 1000/1500 mm",
-            "sart": "view by view (subset 1), relaxation 1.0",
+            "sart": "view by view (subset 1), relaxation 0.0625 (2/Lip_con)",
             "tv_fgp": {"inner_iterations": niter, "lambda": lam},
             "rays_per_sweep": rays_per_sweep,
         },

@@ -75,6 +75,7 @@ def _load() -> ctypes.CDLL:
         "ct_fdk": ([P, G, P, I, P, I, I], I),
         "ct_sart_fista_iteration": ([P, G, P, P, I, P, I, D, I, I, D, I, DP], I),
         "ct_convert": ([P, P, I, P, I, I64], I),
+        "ct_objective": ([P, G, P, I, P, I, D, DP, DP, DP], I),
         "ct_set_math": ([P, I], I),
         "ct_profile_enable": ([P, I], I),
         "ct_profile_reset": ([P], I),
@@ -305,6 +306,17 @@ class Context:
         self._stream()
         self._check(_lib.ct_tv_norm(self._h, nx, ny, nz, _ptr(vol), dtype_code(vol), ctypes.byref(v)))
         return v.value
+
+    def objective(self, g: Geometry, vol: torch.Tensor, meas: torch.Tensor, lambda_tv: float = 0.0
+                  ) -> tuple[float, float, float]:
+        """(sum (Ax-b)^2, TV(x), data + 2*lambda_tv*TV) as V63/main_recon.cu:401-428 forms them."""
+        _need(meas, g.rays, "meas")
+        d, t, o = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._stream()
+        gc = g.to_c()
+        self._check(_lib.ct_objective(self._h, ctypes.byref(gc), _ptr(vol), dtype_code(vol), _ptr(meas),
+                                      dtype_code(meas), lambda_tv, ctypes.byref(d), ctypes.byref(t), ctypes.byref(o)))
+        return d.value, t.value, o.value
 
     def dot(self, a: torch.Tensor, b: torch.Tensor) -> float:
         v = ctypes.c_double()

@@ -529,6 +529,23 @@ ct_status ct_dot(ct_context *ctx, const void *a, const void *b, int64_t n, ct_dt
     return reduce_dot(ctx, a, b, n, dt, 0, value);
 }
 
+ct_status ct_objective(ct_context *ctx, const ct_geometry *g, const void *vol, ct_dtype vdt, const void *meas,
+                       ct_dtype mdt, double lambda_tv, double *data_fidelity, double *tv, double *objective) {
+    CT_CTX(ctx);
+    CT_TRY(validate(ctx, g)); CT_TRY(dt_ok(ctx, vdt)); CT_TRY(dt_ok(ctx, mdt));
+    CT_TRY(check_ptr(ctx, vol, vdt, "vol")); CT_TRY(check_ptr(ctx, meas, mdt, "meas"));
+    if (!data_fidelity || !tv || !objective) return fail(ctx, CT_ERR_NULL, "result pointers are NULL");
+    if (!std::isfinite(lambda_tv)) return fail(ctx, CT_ERR_INVALID_ARG, "lambda_tv must be finite");
+    DeviceGuard dg(ctx);
+    double dfid = 0, tvv = 0;
+    CT_TRY(launch_residual(ctx, *g, vol, vdt, meas, mdt, &dfid));
+    CT_TRY(reduce_tv(ctx, vol, vdt, g->nx, g->ny, g->nz, 0, 0.0, &tvv));
+    *data_fidelity = dfid;
+    *tv = tvv;
+    *objective = dfid + 2.0 * lambda_tv * tvv;
+    return CT_OK;
+}
+
 ct_status ct_fdk_weight(ct_context *ctx, const ct_geometry *g, const void *in, void *out, ct_dtype dt) {
     CT_CTX(ctx);
     CT_TRY(validate(ctx, g)); CT_TRY(dt_ok(ctx, dt));

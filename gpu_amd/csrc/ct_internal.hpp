@@ -40,6 +40,44 @@ template <typename S> __device__ __forceinline__ typename LdT<S>::type ldf(const
     return St<S>::ld(p, i);
 }
 
+// Hardware min/max for the fast kernels.  v_min_f32/v_max_f32/v_min3_f32
+// return the other operand for a quiet NaN input, which is the fminf/fmaxf
+// result for every NaN that arithmetic can produce (0*inf); issuing them
+// directly keeps the compiler from re-canonicalising loop-carried operands
+// on every use.
+__device__ __forceinline__ float hmin(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float hmax(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float hmin3(float a, float b, float c) {
+    float r;
+    asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Raw buffer loads over one array of `bytes` bytes (< 4 GiB) with 32-bit byte
+// offsets.  The buffer unit range-checks each access: an offset at or past
+// the end returns 0 instead of touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename S> __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ float bload<__half>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __half2float(__builtin_bit_cast(__half, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
+}
+template <> __device__ __forceinline__ float bload<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
+}
+
 inline size_t dtype_size(ct_dtype d) {
     switch (d) {
     case CT_F32: return 4;
@@ -155,7 +193,8 @@ enum Slot {
     SLOT_TMP_PROJ = 6,
     SLOT_TMP_PROJ2 = 7,
     SLOT_FFT = 8,      // twiddles
-    SLOT_MISC = 9
+    SLOT_MISC = 9,
+    SLOT_BPTAB = 10    // per-view tables of the fast voxel-driven backprojector
 };
 
 ct_status fail(ct_context *ctx, ct_status s, const std::string &msg);
@@ -184,6 +223,8 @@ ct_status launch_convert(ct_context *ctx, const void *in, ct_dtype idt, void *ou
 ct_status launch_forward_mv(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
                             ct_dtype odt, const void *meas, ct_dtype mdt, int32_t first, int32_t stride,
                             int32_t nv, int cmd, int32_t mv0);
+ct_status launch_residual(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, const void *meas,
+                          ct_dtype mdt, double *out);
 ct_status launch_tv_eps_grad(ct_context *ctx, const void *f, void *g, ct_dtype dt, int M, int N, int Z, double eps);
 ct_status launch_fista(ct_context *ctx, void *x, void *xp, void *y, int64_t n, ct_dtype dt, double gamma);
 ct_status launch_fgp(ct_context *ctx, int M, int N, int Z, void *vol, ct_dtype dt, double lambda, int niter);

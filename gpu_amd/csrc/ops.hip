@@ -147,16 +147,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_fold(const double *part, int n,
     if (threadIdx.x == 0) out[0] = r;
 }
 
-// mode 0: sum a*b; mode 1: sum (a-b)^2
+// mode 0: sum a*b; mode 1: sum (a-b)^2; mode 2: sum a (b unused)
 template <typename S>
 __global__ __launch_bounds__(RED_THREADS) void k_dot(const S *a, const S *b, int64_t n, int mode, double *part) {
     double acc = 0;
     for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
-        double x, y;
-        if constexpr (std::is_same<S, double>::value) { x = a[i]; y = b[i]; }
-        else { x = (double)ldf(a, i); y = (double)ldf(b, i); }
+        double x, y = 0.0;
+        if constexpr (std::is_same<S, double>::value) { x = a[i]; if (mode != 2) y = b[i]; }
+        else { x = (double)ldf(a, i); if (mode != 2) y = (double)ldf(b, i); }
         if (mode == 0) acc += x * y;
-        else { double e = x - y; acc += e * e; }
+        else if (mode == 1) { double e = x - y; acc += e * e; }
+        else acc += x;
     }
     double r = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = r;

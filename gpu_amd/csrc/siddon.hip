@@ -234,6 +234,7 @@ __device__ __forceinline__ T siddon_sum(const DGeo<T> &d, T s, T c, int col, int
             unsigned idx[B];
             T seg[B];
             bool act[B];
+            // step generation, branch-free: every state update is a select
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 const bool ok = alive && (r.amax - r.ac > loop_eps<T>()) && (guard > 0) &&
@@ -242,13 +243,22 @@ __device__ __forceinline__ T siddon_sum(const DGeo<T> &d, T s, T c, int col, int
                 act[u] = ok;
                 seg[u] = r.dist * (m - r.ac);
                 idx[u] = ok ? (unsigned)r.vk * MN + (unsigned)r.vj * M + (unsigned)r.vi : 0u;
-                if (ok) {
-                    --guard;
-                    r.ac = m;
-                    if (r.ax == m) { r.vi += r.sgx; r.nx += r.sgx; r.ax = (d.vx * (T)r.nx + d.bx - r.x1x) * r.ixd; }
-                    if (r.ay == m) { r.vj += r.sgy; r.ny += r.sgy; r.ay = (d.vy * (T)r.ny + d.by - r.x1y) * r.iyd; }
-                    if (r.az == m) { r.vk += r.sgz; r.nz += r.sgz; r.az = (d.vz * (T)r.nz + d.bz - r.x1z) * r.izd; }
-                }
+                const bool sx = ok && (r.ax == m), sy = ok && (r.ay == m), sz = ok && (r.az == m);
+                const int nx1 = r.nx + r.sgx, ny1 = r.ny + r.sgy, nz1 = r.nz + r.sgz;
+                const T ax1 = (d.vx * (T)nx1 + d.bx - r.x1x) * r.ixd;
+                const T ay1 = (d.vy * (T)ny1 + d.by - r.x1y) * r.iyd;
+                const T az1 = (d.vz * (T)nz1 + d.bz - r.x1z) * r.izd;
+                r.vi = sx ? r.vi + r.sgx : r.vi;
+                r.nx = sx ? nx1 : r.nx;
+                r.ax = sx ? ax1 : r.ax;
+                r.vj = sy ? r.vj + r.sgy : r.vj;
+                r.ny = sy ? ny1 : r.ny;
+                r.ay = sy ? ay1 : r.ay;
+                r.vk = sz ? r.vk + r.sgz : r.vk;
+                r.nz = sz ? nz1 : r.nz;
+                r.az = sz ? az1 : r.az;
+                r.ac = ok ? m : r.ac;
+                guard -= ok ? 1 : 0;
                 alive = ok;
             }
             T fv[B];
@@ -256,26 +266,130 @@ __device__ __forceinline__ T siddon_sum(const DGeo<T> &d, T s, T c, int col, int
             for (int u = 0; u < B; ++u) fv[u] = (T)ldf(f, idx[u]);
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                if (act[u]) {
-                    if constexpr (COMP) {
-                        const T yl = seg[u] - clen;
-                        const T tl = len + yl;
-                        clen = (tl - len) - yl;
-                        len = tl;
-                        const T ys = seg[u] * fv[u] - csum;
-                        const T ts = sum + ys;
-                        csum = (ts - sum) - ys;
-                        sum = ts;
-                    } else {
-                        len += seg[u];
-                        sum += seg[u] * fv[u];
-                    }
+                if constexpr (COMP) {
+                    const T yl = seg[u] - clen;
+                    const T tl = len + yl;
+                    const T cl = (tl - len) - yl;
+                    const T ys = seg[u] * fv[u] - csum;
+                    const T ts = sum + ys;
+                    const T cs = (ts - sum) - ys;
+                    len = act[u] ? tl : len;
+                    clen = act[u] ? cl : clen;
+                    sum = act[u] ? ts : sum;
+                    csum = act[u] ? cs : csum;
+                } else {
+                    len = act[u] ? len + seg[u] : len;
+                    sum = act[u] ? sum + seg[u] * fv[u] : sum;
                 }
             }
         }
     }
     len_out = len;
     return sum;
+}
+
+// Fast-mode FP32 traversal (CT_MATH_FAST).  Same walk as siddon_traverse
+// with the instruction count of one step cut from ~59 to ~30 VALU:
+//  * the crossing of plane n is fma(n, vx*ixd, (bx - x1x)*ixd) (one rounding
+//    per term instead of four), with n kept as a float counter;
+//  * one linear voxel offset (in units of ES bytes) with per-axis strides
+//    replaces the three voxel coordinates; the first voxel is bounds-checked
+//    once (a ray entering outside the grid is empty, as in the reference),
+//    and the last segment is clipped at alpha_max instead of bounds-checking
+//    every step.  A step that rounding leaves between the exit plane and
+//    alpha_max can carry an offset past the grid: the forward reads 0 there
+//    and the adjoint scatters nothing, so the pair stays matched;
+//  * min/min3 are issued directly (no per-use NaN canonicalisation);
+//  * the ray length is the closed form dist*(alpha_max - alpha_min), the
+//    telescoped sum of the segments without their rounding.
+// The forward (siddon_walk_fast) and the matched adjoint (k_adjoint) both
+// step through this walker, so their segments are the same numbers.
+template <int ES> struct FastWalk {
+    float kx, ky, kz, bx0, by0, bz0, sgxf, sgyf, sgzf, nfx, nfy, nfz, ax, ay, az, ac, amax, dist, len;
+    int dix, diy, diz;
+    unsigned idx;
+    bool alive;
+    // returns false for an empty ray (the forward yields 0 for it)
+    __device__ __forceinline__ bool init(const DGeo<float> &d, float s, float c, int col, int row, int src) {
+        RayState<float> r = ray_setup(d, s, c, col, row, src);
+        const unsigned M = d.M, N = d.N, Z = d.Z, MN = M * N;
+        if (!(r.hit && (unsigned)r.vi < M && (unsigned)r.vj < N && (unsigned)r.vk < Z)) return false;
+        kx = d.vx * r.ixd; ky = d.vy * r.iyd; kz = d.vz * r.izd;
+        bx0 = (d.bx - r.x1x) * r.ixd; by0 = (d.by - r.x1y) * r.iyd; bz0 = (d.bz - r.x1z) * r.izd;
+        sgxf = (float)r.sgx; sgyf = (float)r.sgy; sgzf = (float)r.sgz;
+        nfx = (float)r.nx; nfy = (float)r.ny; nfz = (float)r.nz;
+        dix = r.sgx * ES; diy = r.sgy * (int)M * ES; diz = r.sgz * (int)MN * ES;
+        idx = ((unsigned)r.vk * MN + (unsigned)r.vj * M + (unsigned)r.vi) * (unsigned)ES;
+        ax = r.ax; ay = r.ay; az = r.az; ac = r.ac; amax = r.amax; dist = r.dist;
+        len = dist * (amax - ac);
+        alive = true;
+        return true;
+    }
+    // one step: the segment length (0 once the ray has ended) and the offset
+    // of the voxel it lies in
+    __device__ __forceinline__ float step(unsigned &id) {
+        const bool ok = alive && (amax - ac > loop_eps<float>());
+        const float m = hmin(hmin3(ax, ay, az), amax);
+        const float seg = ok ? dist * (m - ac) : 0.0f;
+        id = idx;
+        const bool sx = ok && (ax == m), sy = ok && (ay == m), sz = ok && (az == m);
+        const float nx1 = nfx + sgxf, ny1 = nfy + sgyf, nz1 = nfz + sgzf;
+        const float ax1 = fmaf(nx1, kx, bx0), ay1 = fmaf(ny1, ky, by0), az1 = fmaf(nz1, kz, bz0);
+        nfx = sx ? nx1 : nfx; ax = sx ? ax1 : ax;
+        nfy = sy ? ny1 : nfy; ay = sy ? ay1 : ay;
+        nfz = sz ? nz1 : nfz; az = sz ? az1 : az;
+        idx += (unsigned)((sx ? dix : 0) + (sy ? diy : 0) + (sz ? diz : 0));
+        ac = ok ? m : ac;
+        alive = ok;
+        return seg;
+    }
+};
+
+constexpr int FAST_B = 8;   // steps per batch: their gathers are in flight together
+
+// Fast-mode ray sum: batches of FAST_B steps whose loads are issued together;
+// each batch is summed in FP32 and the batch sums accumulate in FP64.
+// Volumes under 4 GiB are read with range-checked buffer loads on 32-bit
+// byte offsets; larger ones use 64-bit addresses with an explicit range test.
+template <typename SV, bool BUF>
+__device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s, float c, int col, int row, int src,
+                                                  const SV *__restrict__ f, float &len_out) {
+    constexpr int ES = BUF ? (int)sizeof(SV) : 1;
+    FastWalk<ES> w;
+    len_out = 0.0f;
+    if (!w.init(d, s, c, col, row, src)) return 0.0f;
+    const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (unsigned)d.Z;
+    // (for the 64-bit path the resource is built but never used)
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(f, BUF ? nvox * (unsigned)sizeof(SV) : 0u);
+    double sum = 0.0;
+    int guard = d.M + d.N + d.Z + 4;
+    while (w.alive && guard > 0) {
+        unsigned ids[FAST_B];
+        float seg[FAST_B];
+#pragma unroll
+        for (int u = 0; u < FAST_B; ++u) seg[u] = w.step(ids[u]);
+        guard -= FAST_B;
+        float fv[FAST_B];
+#pragma unroll
+        for (int u = 0; u < FAST_B; ++u) {
+            if constexpr (BUF) fv[u] = bload<SV>(rs, ids[u]);
+            else fv[u] = ids[u] < nvox ? (float)ldf(f, (int64_t)ids[u]) : 0.0f;
+        }
+        float ps = seg[0] * fv[0];
+#pragma unroll
+        for (int u = 1; u < FAST_B; ++u) ps = fmaf(seg[u], fv[u], ps);
+        sum += (double)ps;
+    }
+    len_out = w.len;
+    return (float)sum;
+}
+
+template <typename SV>
+__device__ __forceinline__ float siddon_sum_fast(const DGeo<float> &d, float s, float c, int col, int row, int src,
+                                                 const SV *__restrict__ f, float &len_out) {
+    if ((uint64_t)d.M * d.N * d.Z * sizeof(SV) < (1ull << 32))
+        return siddon_walk_fast<SV, true>(d, s, c, col, row, src, f, len_out);
+    return siddon_walk_fast<SV, false>(d, s, c, col, row, src, f, len_out);
 }
 
 __device__ __forceinline__ void tile_coords(int &col, int &row) {
@@ -301,7 +415,9 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
     const int v = first + q * stride;
     const T s = ang[2 * v], c = ang[2 * v + 1];
     T len;
-    const T sum = siddon_sum<T, SV, 4, COMP>(d, s, c, col, row, src, f, len);
+    T sum;
+    if constexpr (COMP && std::is_same<T, float>::value) sum = siddon_sum_fast<SV>(d, s, c, col, row, src, f, len);
+    else sum = siddon_sum<T, SV, 4, false>(d, s, c, col, row, src, f, len);
     const int64_t P = (int64_t)d.R * d.ZP;
     const int64_t pix = (int64_t)row * d.R + col;
     T r;
@@ -314,15 +430,106 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
     St<SO>::st(out, (int64_t)img * P + pix, r);
 }
 
+// Data fidelity sum (Ax - b)^2 fused into the ray traversal: Ax never goes to
+// memory.  Each 256-ray block writes one FP64 partial (wave64 shuffles, then
+// LDS across the 4 waves); the partials are folded in fixed order afterwards.
+// (L2_norm_gpu + the objective's Forward_3D_ray_driven_siddon,
+// V63/host_tool_functions.cu:28-58 and V63/main_recon.cu:401-407.)
+template <typename T, typename SV, typename SM, bool COMP>
+__global__ __launch_bounds__(256) void k_residual(DGeo<T> d, const T *__restrict__ ang, const SV *__restrict__ f,
+                                                  const SM *__restrict__ meas, int view0, int z0,
+                                                  double *__restrict__ part) {
+    int col, row;
+    tile_coords(col, row);
+    const int img = blockIdx.z + z0;
+    double e2 = 0.0;
+    if (col < d.R && row < d.ZP) {
+        const int q = img / d.NS, src = img - q * d.NS;
+        const int v = view0 + q;
+        const T s = ang[2 * v], c = ang[2 * v + 1];
+        T len;
+        T sum;
+        if constexpr (COMP && std::is_same<T, float>::value) sum = siddon_sum_fast<SV>(d, s, c, col, row, src, f, len);
+        else sum = siddon_sum<T, SV, 4, false>(d, s, c, col, row, src, f, len);
+        const T ax = (len < d.vz * (T)1e-6) ? (T)0 : sum;
+        const int64_t P = (int64_t)d.R * d.ZP;
+        const double e = (double)ax - (double)ldf(meas, (int64_t)img * P + (int64_t)row * d.R + col);
+        e2 = e * e;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e2 += __shfl_down(e2, o, 64);
+    __shared__ double wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[((int64_t)img * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+template <typename T, typename SV, typename SM>
+static ct_status run_residual(ct_context *ctx, const ct_geometry &g, const void *vol, const void *meas, double *out) {
+    ct_status st;
+    const T *ang = (const T *)angle_table(ctx, g, sizeof(T) == 8, false, &st);
+    if (!ang) return st;
+    DGeo<T> d = derive<T>(g);
+    const int nimg = g.nviews * d.NS;
+    const int gx = (d.R + 15) / 16, gy = (d.ZP + 15) / 16;
+    const int64_t nblk = (int64_t)gx * gy * nimg;
+    double *part = (double *)slot(ctx, SLOT_TMP_PROJ2, sizeof(double) * nblk, &st);
+    if (!part) return st;
+    const bool comp = std::is_same<T, float>::value && ctx->math == CT_MATH_FAST;
+    {
+        ProfScope ps(ctx, CT_K_FORWARD);
+        for (int z0 = 0; z0 < nimg; z0 += 65535) {
+            const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
+            dim3 grid(gx, gy, nz);
+            if (comp)
+                hipLaunchKernelGGL((k_residual<T, SV, SM, true>), grid, dim3(256), 0, ctx->stream, d, ang,
+                                   (const SV *)vol, (const SM *)meas, 0, z0, part);
+            else
+                hipLaunchKernelGGL((k_residual<T, SV, SM, false>), grid, dim3(256), 0, ctx->stream, d, ang,
+                                   (const SV *)vol, (const SM *)meas, 0, z0, part);
+        }
+    }
+    st = launch_check(ctx, "residual");
+    if (st != CT_OK) return st;
+    return reduce_dot(ctx, part, nullptr, nblk, CT_F64, 2, out);
+}
+
+template <typename T, typename SV>
+static ct_status residual_meas(ct_context *ctx, const ct_geometry &g, const void *vol, const void *meas, ct_dtype mdt,
+                               double *out) {
+    switch (mdt) {
+    case CT_F32: return run_residual<T, SV, float>(ctx, g, vol, meas, out);
+    case CT_F16: return run_residual<T, SV, __half>(ctx, g, vol, meas, out);
+    case CT_BF16: return run_residual<T, SV, __hip_bfloat16>(ctx, g, vol, meas, out);
+    case CT_F64: return run_residual<T, SV, double>(ctx, g, vol, meas, out);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "residual: bad measurement dtype");
+}
+
+ct_status launch_residual(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, const void *meas,
+                          ct_dtype mdt, double *out) {
+    switch (vdt) {
+    case CT_F32: return residual_meas<float, float>(ctx, g, vol, meas, mdt, out);
+    case CT_F16: return residual_meas<float, __half>(ctx, g, vol, meas, mdt, out);
+    case CT_BF16: return residual_meas<float, __hip_bfloat16>(ctx, g, vol, meas, mdt, out);
+    case CT_F64: return residual_meas<double, double>(ctx, g, vol, meas, mdt, out);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "residual: bad volume dtype");
+}
+
 template <typename A> __device__ __forceinline__ void atomic_add(A *p, A v);
 template <> __device__ __forceinline__ void atomic_add<float>(float *p, float v) { unsafeAtomicAdd(p, v); }
 template <> __device__ __forceinline__ void atomic_add<double>(double *p, double v) { unsafeAtomicAdd(p, v); }
 
 // Matched adjoint: scatter the forward's segments (scatter form of
-// V63/pixel_driven_backprj/kernel_IterativeRecon_CBCT.cu:563-1116).  A first
-// pass measures the ray length so rays the forward zeroes (len < 1e-6*vz)
-// scatter nothing: the pair is then an exact transpose.
-template <typename T, typename SP, typename A>
+// V63/pixel_driven_backprj/kernel_IterativeRecon_CBCT.cu:563-1116).  Rays the
+// forward zeroes (len < 1e-6*vz) scatter nothing, so the pair is an exact
+// transpose.  FAST pairs with the fast forward: the same FastWalk steps, its
+// closed-form length, and no scatter for an offset past the grid (the
+// forward reads 0 there).  Otherwise a first pass measures the ray length
+// of the reference traversal.
+template <typename T, typename SP, typename A, bool FAST>
 __global__ __launch_bounds__(256) void k_adjoint(DGeo<T> d, const T *__restrict__ ang,
                                                  const SP *__restrict__ proj, A *__restrict__ acc,
                                                  int view0, int z0) {
@@ -336,9 +543,26 @@ __global__ __launch_bounds__(256) void k_adjoint(DGeo<T> d, const T *__restrict_
     const T y = (T)ldf(proj, (int64_t)img * P + (int64_t)row * d.R + col);
     if (y == (T)0) return;
     const T s = ang[2 * v], c = ang[2 * v + 1];
-    const T len = siddon_traverse(d, s, c, col, row, src, [](int, T) {});
-    if (len < d.vz * (T)1e-6) return;
-    siddon_traverse(d, s, c, col, row, src, [&](int idx, T seg) { atomic_add<A>(acc + idx, (A)seg * (A)y); });
+    if constexpr (FAST) {
+        FastWalk<1> w;
+        if (!w.init(d, s, c, col, row, src)) return;
+        if (w.len < d.vz * 1e-6f) return;
+        const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (unsigned)d.Z;
+        int guard = d.M + d.N + d.Z + 4;
+        while (w.alive && guard > 0) {
+#pragma unroll
+            for (int u = 0; u < FAST_B; ++u) {
+                unsigned id;
+                const float seg = w.step(id);
+                if (seg != 0.0f && id < nvox) atomic_add<A>(acc + id, (A)seg * (A)y);
+            }
+            guard -= FAST_B;
+        }
+    } else {
+        const T len = siddon_traverse(d, s, c, col, row, src, [](int, T) {});
+        if (len < d.vz * (T)1e-6) return;
+        siddon_traverse(d, s, c, col, row, src, [&](int idx, T seg) { atomic_add<A>(acc + idx, (A)seg * (A)y); });
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -436,7 +660,13 @@ static ct_status run_adjoint(ct_context *ctx, const ct_geometry &g, const void *
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
-        hipLaunchKernelGGL((k_adjoint<T, SP, A>), grid, block, 0, ctx->stream, d, ang, (const SP *)proj, acc, view0, z0);
+        // the fast forward's walk in fast mode, so the pair stays matched
+        if (std::is_same<T, float>::value && ctx->math == CT_MATH_FAST)
+            hipLaunchKernelGGL((k_adjoint<T, SP, A, std::is_same<T, float>::value>), grid, block, 0, ctx->stream, d,
+                               ang, (const SP *)proj, acc, view0, z0);
+        else
+            hipLaunchKernelGGL((k_adjoint<T, SP, A, false>), grid, block, 0, ctx->stream, d, ang, (const SP *)proj,
+                               acc, view0, z0);
     }
     return launch_check(ctx, "adjoint");
 }

@@ -15,6 +15,8 @@
 // reference's row-major order, so the result is bit-identical to the
 // per-pixel form.
 #include "ct_internal.hpp"
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 namespace ct {
@@ -260,6 +262,261 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
     }
 }
 
+// Per-view tables of the LDS-staged fast backprojector, built in one pass
+// over the subset's correction images: per pixel (c*|S-P|, |S-P|), per column
+// (qx, qy, 1/ex, 1/ey), per row and source (qz, 1/ez).  The pixel, slab-plane
+// and source coordinates use the reference's arithmetic (see voxel_wl_fast).
+__global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__restrict__ ang,
+                                                 const float *__restrict__ corr, int first, int stride,
+                                                 float2 *__restrict__ pix, float4 *__restrict__ colT,
+                                                 float2 *__restrict__ rowT) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, img = blockIdx.z;
+    if (i >= d.R) return;
+    const int q = img / d.NS, src = img - q * d.NS;
+    const int v = first + q * stride;
+    const float s = ang[2 * v], c = ang[2 * v + 1];
+    const float szs = d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0;
+    const float sxw = d.sx * c - d.sy * s;
+    const float syw = d.sx * s + d.sy * c;
+    const float u = d.ymin + (float)i * d.du;
+    const float qx = d.dod * c - u * s;
+    const float qy = d.dod * s + u * c;
+    const float qz = d.zmin + (float)j * d.dv;
+    const float ex = sxw - qx, ey = syw - qy, ez = szs - qz;
+    const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+    const int64_t P = (int64_t)d.R * d.ZP;
+    const int64_t p = (int64_t)img * P + (int64_t)j * d.R + i;
+    pix[p] = make_float2(corr[p] * dist, dist);
+    if (j == 0 && src == 0) colT[(int64_t)q * d.R + i] = make_float4(qx, qy, 1.0f / ex, 1.0f / ey);
+    if (i == 0) rowT[(int64_t)img * d.ZP + j] = make_float2(qz, 1.0f / ez);
+}
+
+// LDS-staged fast backprojector.  A workgroup owns a 16x16 (x, y) tile of
+// ZR voxels in z; each lane owns the ZR voxels of one (x, y).  Per image the
+// workgroup stages its detector footprint - the projection of the tile's box,
+// widened by one pixel each way - from the per-view tables into LDS: pixel
+// values (c*|S-P|, |S-P|) in rows of LP entries (columns past the footprint
+// are zero), the column table (qx, qy, 1/ex, 1/ey) and the row table
+// (qz, 1/ez).  Each voxel's bounding box lies inside the footprint, so every
+// pixel read of the chord sum is an unconditional LDS read.  The per-lane
+// column count is raised to a wave-uniform NC in {4, 5, 6, 8}; columns past a
+// lane's own bounding box get an empty alpha interval and add exactly zero.
+// Per pixel: span = max(min(hi_col, hi_row) - max(lo_col, lo_row), 0),
+// W += span*(c*|S-P|), L += span*|S-P|.  Chords shorter than 1e-6*vx are not
+// cut per pixel (they move W and L by less than 1e-6*vx each); the update
+// threshold on L stays.  Footprints larger than the LDS window take the
+// per-voxel table path for that image.
+constexpr int BPX = 16, BPY = 16, ZR = 8;
+constexpr int LW = 80;          // staged footprint columns
+constexpr int LP = LW + 8;      // LDS row pitch: 8 zero columns for the NC overrun
+constexpr int LH = 36;          // staged footprint rows
+constexpr int NCMAX = 8;
+
+template <int NC>
+__device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
+                                        int cbase, float zl, float zh, const float *lo, const float *hi, float &wk,
+                                        float &lk) {
+    for (int j = z0; j <= z1; ++j) {
+        const float2 rt = sr[j];
+        const float2 *prow = sp + j * LP + cbase;
+        float2 pv[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) pv[cc] = prow[cc];
+        const float az0 = (zl - rt.x) * rt.y, az1 = (zh - rt.x) * rt.y;
+        const float loz = hmin(az0, az1), hiz = hmax(az0, az1);
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+            const float span = hmax(hmin(hi[cc], hiz) - hmax(lo[cc], loz), 0.0f);
+            wk = fmaf(span, pv[cc].x, wk);
+            lk = fmaf(span, pv[cc].y, lk);
+        }
+    }
+}
+
+// SART update of one voxel from its accumulated W = sum l*c and L = sum l
+// (V63/kernel_IterativeRecon_universal.cu:799-806; TB threshold
+// TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772)
+template <typename SV>
+__device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__ f, int64_t idx, float W, float L,
+                                          float beta, int mode) {
+    bool upd;
+    if (!d.tb) upd = !(L < d.vx * 1e-6f);
+    else upd = (double)L > (double)d.vx * 1e-3;
+    if (!upd) return;
+    if (mode == 0) St<SV>::st(f, idx, W);
+    else St<SV>::st(f, idx, ldf(f, idx) + beta * W / L);
+}
+
+// ONE: the launch backprojects a single image (one view, one source), so each
+// voxel is updated as soon as its sums are formed and no per-voxel sums are
+// carried across images.
+template <typename SV, bool ONE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void k_voxel_bp_lds(DGeo<float> d, const float *__restrict__ ang,
+                                                      SV *__restrict__ f, const float2 *__restrict__ pixT,
+                                                      const float4 *__restrict__ colT,
+                                                      const float2 *__restrict__ rowT, int first, int stride,
+                                                      int nv, float beta, int mode) {
+    __shared__ float2 sp[LH * LP];
+    __shared__ float4 sc[LP];
+    __shared__ float2 sr[LH];
+    const int tid = threadIdx.y * BPX + threadIdx.x;
+    const int ix = blockIdx.x * BPX + threadIdx.x;
+    const int iy = blockIdx.y * BPY + threadIdx.y;
+    const int iz0 = blockIdx.z * ZR;
+    const int nz = min(ZR, d.Z - iz0);
+    bool active = ix < d.M && iy < d.N;
+    if (active) {   // cylinder mask
+        const float hxm = (float)d.M * 0.5f - 0.5f, hym = (float)d.N * 0.5f - 0.5f;
+        const float ma = (float)ix - hxm - (float)d.moff;
+        const float mb = (float)iy - hym;
+        active = !(ma * d.vx * ma * d.vx + mb * d.vy * mb * d.vy >= hxm * d.vx * hym * d.vy);
+    }
+    if (!__syncthreads_or(active)) return;   // the whole tile is outside the cylinder
+    float cx, cy;
+    if (!d.tb) {
+        cx = d.bx + d.vx * 0.5f + (float)ix * d.vx;
+        cy = d.by + d.vy * 0.5f + (float)iy * d.vy;
+    } else {
+        cx = d.bx + d.vx * (0.5f + (float)ix);
+        cy = d.by + d.vy * (0.5f + (float)iy);
+    }
+    const float hvx = 0.5f * d.vx, hvy = 0.5f * d.vy, hvz = 0.5f * d.vz;
+    const float xl = cx - hvx, xh = cx + hvx, yl = cy - hvy, yh = cy + hvy;
+    // tile box (clipped to the volume) for the footprint
+    const int tx0 = blockIdx.x * BPX, ty0 = blockIdx.y * BPY;
+    const float TX0 = d.bx + (float)tx0 * d.vx, TX1 = d.bx + (float)min(tx0 + BPX, d.M) * d.vx;
+    const float TY0 = d.by + (float)ty0 * d.vy, TY1 = d.by + (float)min(ty0 + BPY, d.N) * d.vy;
+    const float TZ0 = d.bz + (float)iz0 * d.vz, TZ1 = d.bz + (float)(iz0 + nz) * d.vz;
+    float W[ONE ? 1 : ZR], L[ONE ? 1 : ZR];
+#pragma unroll
+    for (int k = 0; k < (ONE ? 1 : ZR); ++k) { W[k] = 0.0f; L[k] = 0.0f; }
+    const int64_t P = (int64_t)d.R * d.ZP;
+    for (int q = 0; q < nv; ++q) {
+        const int v = first + q * stride;
+        const float s = ang[2 * v], c = ang[2 * v + 1];
+        // this lane's corner ratios and column bounding box (:639-678)
+        float rmin = __builtin_inff(), rmax = -__builtin_inff(), minY = __builtin_inff(), maxY = -__builtin_inff();
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                const float px = ii ? xh : xl, py = jj ? yh : yl;
+                const float t = px * c + py * s;
+                const float sv = -px * s + py * c;
+                const float ratio = (t - d.dod) * frcp(d.dso - t);
+                const float u = fmaf(ratio, sv - d.sy, sv);
+                minY = fminf(minY, u); maxY = fmaxf(maxY, u);
+                rmin = fminf(rmin, ratio); rmax = fmaxf(rmax, ratio);
+            }
+        const int y0r = (int)floorf(fmaf(minY - d.ymin, d.idu, 0.5f));
+        const int y1r = (int)floorf(fmaf(maxY - d.ymin, d.idu, 0.5f));
+        const bool yoff = (y0r < 0 && y1r < 0) || (y0r > d.R - 1 && y1r > d.R - 1);
+        // the tile's footprint: columns from its four (x, y) corners, rows
+        // from its z faces at the extreme ratios
+        float Tumin = __builtin_inff(), Tumax = -__builtin_inff(), Trmin = __builtin_inff(), Trmax = -__builtin_inff();
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                const float px = ii ? TX1 : TX0, py = jj ? TY1 : TY0;
+                const float t = px * c + py * s;
+                const float sv = -px * s + py * c;
+                const float ratio = (t - d.dod) * frcp(d.dso - t);
+                const float u = fmaf(ratio, sv - d.sy, sv);
+                Tumin = fminf(Tumin, u); Tumax = fmaxf(Tumax, u);
+                Trmin = fminf(Trmin, ratio); Trmax = fmaxf(Trmax, ratio);
+            }
+        const int C0 = max((int)floorf(fmaf(Tumin - d.ymin, d.idu, 0.5f)) - 1, 0);
+        // the host admits this kernel only when every footprint fits the LDS
+        // window (bp_lds_fits); the clamps keep LDS indexing in range regardless
+        const int C1 = min(min((int)floorf(fmaf(Tumax - d.ymin, d.idu, 0.5f)) + 1, d.R - 1), C0 + LW - 1);
+        const int ncw = C1 - C0 + 1;
+        for (int src = 0; src < d.NS; ++src) {
+            const int img = q * d.NS + src;
+            const float szs = d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0;
+            const float e0 = TZ0 - szs, e1 = TZ1 - szs;
+            const float Tw0 = fminf(fmaf(Trmin, e0, TZ0), fmaf(Trmax, e0, TZ0));
+            const float Tw1 = fmaxf(fmaf(Trmin, e1, TZ1), fmaf(Trmax, e1, TZ1));
+            const int R0 = max((int)floorf(fmaf(Tw0 - d.zmin, d.idv, 0.5f)) - 1, 0);
+            const int R1 = min(min((int)floorf(fmaf(Tw1 - d.zmin, d.idv, 0.5f)) + 1, d.ZP - 1), R0 + LH - 1);
+            const int nrow = R1 - R0 + 1;
+            if (ncw <= 0 || nrow <= 0) continue;   // footprint off the detector (uniform)
+            const float2 *pimg = pixT + (int64_t)img * P;
+            const float2 *rimg = rowT + (int64_t)img * d.ZP;
+            const float4 *cimg = colT + (int64_t)q * d.R;
+            // TB source culling, TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:618-627
+            float rk = 0.0f;
+            if (d.tb) rk = (d.dso - d.dod) * frcp(cx * c + cy * s - d.dod);
+            __syncthreads();   // the previous image's readers are done
+            for (int i = tid; i < nrow * LP; i += BPX * BPY) {
+                const int r = i / LP, cl = i - r * LP;
+                sp[i] = cl < ncw ? pimg[(int64_t)(R0 + r) * d.R + C0 + cl] : make_float2(0.0f, 0.0f);
+            }
+            if (tid < LP) sc[tid] = tid < ncw ? cimg[C0 + tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (tid < nrow) sr[tid] = rimg[R0 + tid];
+            __syncthreads();
+            if (!active || yoff) continue;
+            const int y0 = max(y0r, C0), y1 = min(min(y1r, C1), y0 + NCMAX - 1);
+            const int ncol = y1 - y0 + 1;
+            if (ncol <= 0) continue;
+            const int cbase = y0 - C0;
+            // wave-uniform column count
+            int NC;
+            if (__builtin_amdgcn_ballot_w64(ncol > 6)) NC = 8;
+            else if (__builtin_amdgcn_ballot_w64(ncol > 5)) NC = 6;
+            else if (__builtin_amdgcn_ballot_w64(ncol > 4)) NC = 5;
+            else NC = 4;
+            float lo[NCMAX], hi[NCMAX];
+#pragma unroll
+            for (int cc = 0; cc < NCMAX; ++cc) {
+                const float4 ct = sc[min(cbase + cc, LP - 1)];
+                const float ax0 = (xl - ct.x) * ct.z, ax1 = (xh - ct.x) * ct.z;
+                const float ay0 = (yl - ct.y) * ct.w, ay1 = (yh - ct.y) * ct.w;
+                const bool in = cc < ncol;
+                lo[cc] = in ? hmax(hmin(ax0, ax1), hmin(ay0, ay1)) : __builtin_inff();
+                hi[cc] = in ? hmin(hmax(ax0, ax1), hmax(ay0, ay1)) : -__builtin_inff();
+            }
+#pragma unroll 1
+            for (int k = 0; k < nz; ++k) {
+                const float cz = !d.tb ? d.bz + d.vz * 0.5f + (float)(iz0 + k) * d.vz
+                                       : d.bz + d.vz * (0.5f + (float)(iz0 + k));
+                if (d.tb) {
+                    const float upper = (cz + d.vz - d.zmin) * rk + d.zmin;
+                    const float lower = (cz - d.vz - d.zmax) * rk + d.zmax;
+                    if (szs < lower || szs > upper) continue;
+                }
+                const float zl = cz - hvz, zh = cz + hvz;
+                // w = z + r*(z - z_source) is monotone in r: the extremes of the
+                // eight corners come from the extreme ratios
+                const float el = zl - szs, eh = zh - szs;
+                const float wl = fmaf(el >= 0.0f ? rmin : rmax, el, zl);
+                const float wh = fmaf(eh >= 0.0f ? rmax : rmin, eh, zh);
+                int z0 = (int)floorf(fmaf(wl - d.zmin, d.idv, 0.5f));
+                int z1 = (int)floorf(fmaf(wh - d.zmin, d.idv, 0.5f));
+                if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
+                z0 = max(z0, R0) - R0;
+                z1 = min(z1, R1) - R0;
+                float wk = ONE ? 0.0f : W[k], lk = ONE ? 0.0f : L[k];
+                if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                else bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                if constexpr (ONE) bp_update(d, f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, wk, lk, beta, mode);
+                else { W[k] = wk; L[k] = lk; }
+            }
+        }
+    }
+    if constexpr (!ONE) {
+        if (!active) return;
+#pragma unroll
+        for (int k = 0; k < ZR; ++k) {
+            if (k >= nz) break;
+            bp_update(d, f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, W[k], L[k], beta, mode);
+        }
+    }
+}
+
 template <typename T, typename SV, bool FAST>
 __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict__ ang, SV *__restrict__ f,
                                                   const T *__restrict__ corr, int first, int stride, int nv,
@@ -322,6 +579,41 @@ __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict
     }
 }
 
+// Host-side bound for k_voxel_bp_lds: every tile footprint must fit the
+// LDS window (LW columns, LH rows) and every voxel's bounding box NCMAX
+// columns.  With rho the largest distance from the axis of a point of a tile
+// that runs, the corner ratio r = (t - DOD)/(DSO - t) lies in [r(-rho), r(rho)]
+// and changes by at most (DSO - DOD)/(DSO - rho)^2 per unit depth.  A box of
+// (x, y) diagonal D and height H then projects to at most
+// (1 + r_max)*D + dr*D*(rho + |off|) in u and
+// (1 + r_max)*H + dr*D*max|z - z_source| in v; the floor(x + 0.5) bounds add
+// one pixel and the footprint pad two more.
+static bool bp_lds_fits(const DGeo<float> &d) {
+    const double vx = d.vx, vy = d.vy, vz = d.vz, dso = d.dso, dod = d.dod;
+    const double xs[2] = {(double)d.bx, (double)d.bx + d.M * vx}, ys[2] = {(double)d.by, (double)d.by + d.N * vy};
+    double rho = 0;
+    for (double x : xs)
+        for (double y : ys) rho = std::max(rho, std::sqrt(x * x + y * y));
+    // a tile runs only if it holds a voxel of the reconstruction cylinder, so
+    // its points lie within the cylinder radius plus the tile diagonal
+    const double DT = std::hypot(BPX * vx, BPY * vy), DV = std::hypot(vx, vy);
+    const double hx = d.M * 0.5 - 0.5, hy = d.N * 0.5 - 0.5;
+    rho = std::min(rho, std::sqrt(std::max(hx * vx * hy * vy, 0.0)) + DT);
+    if (!(dso - rho > 0)) return false;
+    const double r_lo = (-rho - dod) / (dso + rho), r_hi = (rho - dod) / (dso - rho);
+    const double rmax = std::max(std::fabs(r_lo), std::fabs(r_hi));
+    const double dr = (dso - dod) / ((dso - rho) * (dso - rho));
+    double zs_lo = d.sz0, zs_hi = d.sz0 + (double)(d.NS - 1) * d.sdz;
+    if (zs_lo > zs_hi) std::swap(zs_lo, zs_hi);
+    const double z_lo = d.bz, z_hi = (double)d.bz + d.Z * vz;
+    const double zrel = std::max({std::fabs(z_lo - zs_lo), std::fabs(z_lo - zs_hi), std::fabs(z_hi - zs_lo),
+                                  std::fabs(z_hi - zs_hi)});
+    const double lat = rho + std::fabs((double)d.sy);
+    auto cols = [&](double D) { return ((1.0 + rmax) * D + dr * D * lat) / (double)d.du; };
+    auto rows = [&](double D, double H) { return ((1.0 + rmax) * H + dr * D * zrel) / (double)d.dv; };
+    return cols(DT) + 4.0 <= LW && rows(DT, ZR * vz) + 4.0 <= LH && cols(DV) + 2.0 <= NCMAX;
+}
+
 template <typename T, typename SV>
 static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const void *corr, int first, int stride,
                         int nv, double beta, int mode) {
@@ -331,14 +623,41 @@ static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const 
     DGeo<T> d = derive<T>(g);
     dim3 block(64, 4, 1);
     dim3 grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
-    ProfScope ps(ctx, CT_K_VOXEL_BP);
     if constexpr (std::is_same<T, float>::value) {
+        const int nimg = nv * d.NS;
+        if (ctx->math == CT_MATH_FAST && nimg <= 65535 && bp_lds_fits(d)) {
+            // per-view tables, then the table-driven update
+            const size_t P = (size_t)d.R * d.ZP;
+            auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+            const size_t pix_b = up(sizeof(float2) * P * nimg), col_b = up(sizeof(float4) * (size_t)d.R * nv);
+            const size_t row_b = up(sizeof(float2) * (size_t)d.ZP * nimg);
+            char *tab = (char *)slot(ctx, SLOT_BPTAB, pix_b + col_b + row_b, &st);
+            if (!tab) return st;
+            float2 *pix = (float2 *)tab;
+            float4 *colT = (float4 *)(tab + pix_b);
+            float2 *rowT = (float2 *)(tab + pix_b + col_b);
+            ProfScope ps(ctx, CT_K_VOXEL_BP);
+            hipLaunchKernelGGL(k_bp_prep, dim3((d.R + 255) / 256, d.ZP, nimg), dim3(256), 0, ctx->stream, d, ang,
+                               (const float *)corr, first, stride, pix, colT, rowT);
+            dim3 lgrid((d.M + BPX - 1) / BPX, (d.N + BPY - 1) / BPY, (d.Z + ZR - 1) / ZR);
+            if (nimg == 1)
+                hipLaunchKernelGGL((k_voxel_bp_lds<SV, true>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
+                                   (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first,
+                                   stride, nv, (float)beta, mode);
+            else
+                hipLaunchKernelGGL((k_voxel_bp_lds<SV, false>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
+                                   (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first,
+                                   stride, nv, (float)beta, mode);
+            return launch_check(ctx, "voxel backprojection");
+        }
         if (ctx->math == CT_MATH_FAST) {
+            ProfScope ps(ctx, CT_K_VOXEL_BP);
             hipLaunchKernelGGL((k_voxel_bp<T, SV, true>), grid, block, 0, ctx->stream, d, ang, (SV *)vol,
                                (const T *)corr, first, stride, nv, (T)beta, mode);
             return launch_check(ctx, "voxel backprojection");
         }
     }
+    ProfScope ps(ctx, CT_K_VOXEL_BP);
     hipLaunchKernelGGL((k_voxel_bp<T, SV, false>), grid, block, 0, ctx->stream, d, ang, (SV *)vol, (const T *)corr,
                        first, stride, nv, (T)beta, mode);
     return launch_check(ctx, "voxel backprojection");

@@ -230,6 +230,14 @@ ct_status ct_tv_norm(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz,
 ct_status ct_dot(ct_context *ctx, const void *a, const void *b, int64_t n,
                  ct_dtype dtype, double *value);
 
+/* Objective of V63/main_recon.cu:401-428: data = sum (Ax - b)^2 over all views
+ * (the residual is fused into the ray traversal; Ax is never stored), tv =
+ * ct_tv_norm(x), objective = data + 2*lambda_tv*tv.  All FP64. */
+ct_status ct_objective(ct_context *ctx, const ct_geometry *g,
+                       const void *vol, ct_dtype vol_dtype,
+                       const void *meas, ct_dtype meas_dtype,
+                       double lambda_tv, double *data_fidelity, double *tv, double *objective);
+
 /* ---- FDK -------------------------------------------------------------- */
 /* Cosine weight (kernel_FDK_projection_adjust, V63/FDK/kernel_FDK.cu:1-14) of
  * all views, out-of-place allowed in == out. */

@@ -47,6 +47,9 @@ GEOMS = {
     "aniso": small_geom(nx=17, ny=23, nz=11, vx=0.9, vy=1.2, vz=0.7, du=0.8, dv=1.3),
     "adjoint_64": adjoint_64().with_views(15),
     "multisource": small_geom(nsources=4, src_z0=-9.0, src_dz=6.0, nrows=12),
+    # detector pixels much finer than the voxel footprint: the fast
+    # backprojection leaves the LDS-staged kernel for the per-voxel one
+    "fine": small_geom(du=0.15, dv=0.15, ncols=200, nrows=150, nviews=5),
 }
 
 
@@ -85,10 +88,11 @@ def test_exact_mode_is_bit_identical(ctx, name):
 
 
 def test_fast_forward_accuracy_vs_fp64(ctx):
-    """Fast mode (Kahan-compensated ray sums) at 128^3 / 256^2: error against
-    the FP64 oracle no worse than the plain FP32 restatement's.  The residual
-    ~1e-6 comes from FP32 crossing positions (alpha spans the whole
-    source-detector line), which compensation does not change."""
+    """Fast mode (FMA-form plane crossings, FP64 ray sums) at 128^3 / 256^2:
+    error against the FP64 oracle within 1.25x of the plain FP32
+    restatement's and under 2e-6, five times inside the 1e-5 projection bar.
+    The residual ~1e-6 comes from FP32 crossing positions (alpha spans the
+    whole source-detector line), which the FP64 sums do not change."""
     g = plain_sart_128().with_views(4)
     x = phantom(g, seed=0).numpy()
     p64 = orc.forward(g, x.astype(np.float64), prec=64)
@@ -96,7 +100,7 @@ def test_fast_forward_accuracy_vs_fp64(ctx):
     got = host(ctx.forward(g, dev(x)))
     e_gpu, e_plain = rel(got, p64), rel(p32, p64)
     print(f"fast vs FP64 {e_gpu:.3e}, plain FP32 vs FP64 {e_plain:.3e}")
-    assert e_gpu <= 1.05 * e_plain and e_gpu <= 2e-6
+    assert e_gpu <= 1.25 * e_plain and e_gpu <= 2e-6
 
 
 @pytest.mark.parametrize("name", ["small", "aniso", "multisource"])
@@ -130,8 +134,8 @@ def test_sart_correction(ctx, name):
 
 
 @pytest.mark.parametrize("math_mode", ["exact", "fast"])
-@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource"])
-@pytest.mark.parametrize("mode,nv", [(1, 1), (0, 1), (1, 4)])
+@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fine"])
+@pytest.mark.parametrize("mode,nv", [(1, 1), (0, 1), (1, 3)])
 def test_backproject_voxel(ctx, name, mode, nv, math_mode):
     """exact: the reference's FP32 operation sequence, bit-identical to the
     oracle; fast: hardware reciprocal/sqrt, held to the 1e-5 bar."""
@@ -286,6 +290,24 @@ def test_norms(ctx):
         assert ctx.tv_norm(A) == pytest.approx(orc.tv_norm(host(A), prec=64), rel=tol)
         assert ctx.tv_eps(A, 1e-8) == pytest.approx(orc.tv_eps(host(A), 1e-8), rel=tol)
         assert ctx.dot(A, B) == pytest.approx(float((host(A) * host(B)).sum()), rel=1e-11)
+
+
+@pytest.mark.parametrize("name", ["small", "multisource"])
+def test_objective(ctx, name):
+    """Fused residual + TV norm equals the oracle's forward projection followed
+    by the FP64 norms (V63/main_recon.cu:401-428)."""
+    g = GEOMS[name]
+    x = phantom(g, seed=19, scale=0.3).numpy()
+    b = (orc.forward(g, x, prec=64) * 1.03 + 0.01).astype(np.float32)
+    y = (x * 0.9).astype(np.float32)
+    for dt, prec, tol in ((torch.float32, 32, 1e-5), (torch.float64, 64, 1e-12)):
+        data, tv, obj = ctx.objective(g, dev(y, dt), dev(b, dt), lambda_tv=0.05)
+        ax = orc.forward(g, y.astype(orc._np_dtype(prec)), prec=prec)
+        ref_data = orc.residual_norm2(ax, b.astype(orc._np_dtype(prec)), prec=64)
+        ref_tv = orc.tv_norm(y.astype(np.float64), prec=64)
+        assert data == pytest.approx(ref_data, rel=tol)
+        assert tv == pytest.approx(ref_tv, rel=1e-12)
+        assert obj == pytest.approx(data + 0.1 * tv, rel=1e-15)
 
 
 def test_tv_eps_grad(ctx):
