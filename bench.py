@@ -8,15 +8,16 @@ Workload (synthetic, seeded): 512^3 volume of 0.5 mm voxels holding the
 with I0 = 1e5 photons per ray, then the log transform.  One step = one outer
 iteration: a SART sweep over all 1024 views in the reference's view-by-view
 order (relaxation 2/32, the reference's 2/Lip_con), TV-FGP with 20 inner
-iterations (lambda 1e-3) and the
-FISTA momentum, all device resident.  Inputs are in HBM before timing starts.
+iterations (lambda 1e-3) and the FISTA momentum, all device resident.
+Inputs are in HBM before timing starts.
 
 value = fwd+back GRays/s of the FP32 path = rays per sweep / (time of the
 Siddon correction kernel + time of the voxel-driven update kernel), both timed
 live with HIP events on the context stream.  The same run reports the FP16
-path, seconds per iteration of both, the roofline of the dominant kernel and
-the CPU oracle (OpenMP restatement of the reference kernels) on a bounded
-sample of the same sweep.
+path, seconds per iteration of both, the roofline of the dominant kernel,
+FDK (Hann) over the same projections, the autograd pair (ray sums + matched
+adjoint) on a sample of views, and the CPU oracle (OpenMP restatement of the
+reference kernels) on a bounded sample of the same sweep.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -34,6 +35,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "fwd+back GRays/s; s per SART-FISTA iter at 512³/1024 views; FP16 vs FP32"
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PER_S = 256 * 2.4e9     # wave64 VALU instructions per second (1 per CU per clock)
 
 
 def log(*a):
@@ -103,7 +105,7 @@ def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
     return 0
 
 
-def cpu_baseline(g, x_host, meas_host, nviews):
+def cpu_baseline(g, meas_host, nviews, beta):
     """Oracle SART correction + voxel update for the first `nviews` views of the
     same sweep, FP32, OpenMP over all host cores."""
     import numpy as np
@@ -113,7 +115,7 @@ def cpu_baseline(g, x_host, meas_host, nviews):
     t0 = time.perf_counter()
     for j in range(nviews):
         c = orc.sart_correction(g, v, meas_host[j:j + 1], view0=j, nviews=1, prec=32)
-        v = orc.backproject_voxel(g, v, c, view0=j, nviews=1, beta=1.0, mode=1, prec=32)
+        v = orc.backproject_voxel(g, v, c, view0=j, nviews=1, beta=beta, mode=1, prec=32)
     dt = time.perf_counter() - t0
     rays = nviews * g.nrows * g.ncols
     return {"value": rays / dt / 1e9, "unit": "GRays/s", "cores": threads, "kind": "port",
@@ -134,8 +136,8 @@ def main() -> int:
                          "oracle), FP32, FP16 and BF16; prints per-iteration RMSE / relative L2 against FP64 and "
                          "the phantom as one JSON line, then exits")
     ap.add_argument("--pmc-sample", type=int, default=0,
-                    help="counter-collection mode: run N views of the sweep kernels at full shape plus a "
-                         "1 GiB device copy (calibration) and exit without the JSON line")
+                    help="counter-collection mode: a SART sweep over the first N views at full shape plus a "
+                         "1 GiB device copy (calibration); exits without the JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -158,7 +160,8 @@ def main() -> int:
         zz, yy, xx = torch.meshgrid(*(torch.arange(n, device="cuda", dtype=torch.float32) - n / 2 + 0.5
                                       for n in g.vol_shape), indexing="ij")
         ball = ((xx * xx + yy * yy + zz * zz) < (0.4 * g.nx) ** 2).to(torch.float32) * 0.02
-        meas = ctx.forward(g, ball, view0=0, nviews=args.pmc_sample)
+        gs = g.with_views(args.pmc_sample)      # the first N views of the orbit
+        meas = ctx.forward(gs, ball)
         # calibration: the library's own element copy (k_convert F32 -> F32) of
         # 2^28 floats, 1 GiB read + 1 GiB written
         src = torch.ones(1 << 28, device="cuda")
@@ -166,9 +169,7 @@ def main() -> int:
         ctx.convert(src, dst)
         torch.cuda.synchronize()
         vol = torch.zeros(g.vol_shape, device="cuda")
-        for j in range(args.pmc_sample):
-            c = ctx.sart_correction(g, vol, meas[j:j + 1], view0=j, nviews=1)
-            ctx.backproject_voxel(g, vol, c, view0=j, nviews=1, beta=beta, mode=1)
+        ctx.sart_sweep(gs, vol, meas, beta=beta, subset=1)      # the sweep's kernels, as timed
         torch.cuda.synchronize()
         log(f"pmc sample done: {args.pmc_sample} views")
         return 0
@@ -199,10 +200,17 @@ def main() -> int:
     dom = max(kernels, key=lambda k: prof32[k][0])
     avg_ms, abytes = kernels[dom]
     achieved = abytes / (avg_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes and VALU instructions per launch of the same kernels, from the
+    # committed rocprofv3 counter passes (they cannot be read from inside this
+    # process); the live time turns the VALU count into an issue-rate fraction
+    traffic, valu_frac = None, None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get(dom)
+        tj = json.loads(tf.read_text())
+        traffic = tj.get("hbm_bytes_per_launch", {}).get(dom)
+        vi = tj.get("valu_insts_per_launch", {}).get(dom)
+        if vi:
+            valu_frac = round(vi / (VALU_ISSUE_PER_S * avg_ms * 1e-3), 4)
 
     result = {
         "metric": METRIC,
@@ -227,8 +235,38 @@ def main() -> int:
         "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof32.items() if v[1]},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": abytes},
+                     "algorithmic_bytes_per_launch": abytes, "valu_issue_frac": valu_frac},
     }
+
+    # secondary figures on the same inputs: FDK (Hann) over all views, and the
+    # autograd pair (ray sums + matched adjoint) on a sample of views
+    ctx.profile(True, reset=True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    vol_fdk = ctx.fdk(g, meas32, window="hann")
+    e1.record()
+    torch.cuda.synchronize()
+    fdk_s = e0.elapsed_time(e1) / 1e3
+    pf = ctx.profile_read()
+    result["fdk"] = {"s": round(fdk_s, 4), "window": "hann", "views": g.nviews,
+                     "kernels_ms": {k: round(v[0], 3) for k, v in pf.items() if v[1]}}
+    log(f"FDK (Hann, {g.nviews} views): {fdk_s:.3f} s")
+    del vol_fdk
+    nsamp = 32
+    ctx.profile(True, reset=True)
+    for j in range(nsamp):
+        pj = ctx.forward(g, x32, view0=j * (g.nviews // nsamp), nviews=1)
+        ctx.adjoint(g, pj, view0=j * (g.nviews // nsamp), nviews=1)
+    torch.cuda.synchronize()
+    pa = ctx.profile_read()
+    t_pair = (pa["forward"][0] + pa["adjoint"][0]) / nsamp   # ms per view
+    result["autograd_pair"] = {
+        "grays": round(g.nrows * g.ncols / (t_pair * 1e-3) / 1e9, 4),
+        "forward_ms": round(pa["forward"][0] / nsamp, 4), "adjoint_ms": round(pa["adjoint"][0] / nsamp, 4),
+        "sample": f"{nsamp} views spread over the orbit, ray sums then the matched adjoint (FP64 accumulation)"}
+    log(f"autograd pair: forward {pa['forward'][0] / nsamp:.3f} ms + adjoint {pa['adjoint'][0] / nsamp:.3f} ms per view")
+    ctx.profile(False)
 
     if not args.no_fp16:
         log("FP16 path")
@@ -248,7 +286,7 @@ def main() -> int:
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_views} views")
         meas_host = meas32[: args.cpu_views].cpu().numpy()
-        result["cpu_baseline"] = cpu_baseline(g, None, meas_host, args.cpu_views)
+        result["cpu_baseline"] = cpu_baseline(g, meas_host, args.cpu_views, beta)
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

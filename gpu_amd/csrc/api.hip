@@ -176,6 +176,28 @@ static ct_status sweep(ct_context *ctx, const ct_geometry &g, void *vol, ct_dtyp
     if (!corr) return st;
     const ct_dtype cdt = dbl ? CT_F64 : CT_F32;
     const int nv = g.nviews;
+    if (sweep_zq_eligible(ctx, g, vdt, subset)) {
+        // fast mode: the sweep runs on a z-quad-layout copy of the volume
+        // (coalesced gathers for the ray walk), converted once each way
+        const size_t bytes = (size_t)g.nx * g.ny * (size_t)((g.nz + 3) & ~3) * dtype_size(vdt);
+        void *volq = slot(ctx, SLOT_ZQVOL, bytes, &st);
+        if (!volq) return st;
+        if ((st = launch_zq_pack(ctx, vol, vdt, volq, g.nx, g.ny, g.nz)) != CT_OK) return st;
+        auto step = [&](int first, int stride, int cnt) {
+            ct_status s2 = launch_correction_zq(ctx, g, volq, vdt, (float *)corr, meas, mdt, first, stride, cnt);
+            if (s2 != CT_OK) return s2;
+            return launch_voxel_bp_zq(ctx, g, volq, vdt, corr, first, stride, cnt, beta);
+        };
+        if (order == CT_ORDER_INTERLEAVED && subset > 1) {
+            const int nsub = (nv + subset - 1) / subset;
+            for (int s = 0; s < nsub; ++s)
+                if ((st = step(s, nsub, (nv - s + nsub - 1) / nsub)) != CT_OK) return st;
+        } else {
+            for (int p0 = 0; p0 < nv; p0 += subset)
+                if ((st = step(p0, 1, std::min(subset, nv - p0))) != CT_OK) return st;
+        }
+        return launch_zq_unpack(ctx, volq, vdt, vol, g.nx, g.ny, g.nz);
+    }
     if (order == CT_ORDER_INTERLEAVED && subset > 1) {
         const int nsub = (nv + subset - 1) / subset;
         for (int s = 0; s < nsub; ++s) {

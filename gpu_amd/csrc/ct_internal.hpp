@@ -61,6 +61,18 @@ __device__ __forceinline__ float hmin3(float a, float b, float c) {
     return r;
 }
 
+// Workgroups are dealt to MI355X's 8 XCDs round-robin by linear id.  This
+// remap hands each XCD one contiguous eighth of the tile range (x fastest,
+// then y, then z), so the tiles that share an XCD's L2 are neighbours.
+__device__ __forceinline__ int3 xcd_tile() {
+    const unsigned gx = gridDim.x, gy = gridDim.y, T = gx * gy * gridDim.z;
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned t = (T & 7u) == 0 ? (L & 7u) * (T >> 3) + (L >> 3) : L;
+    const unsigned z = t / (gx * gy), r = t - z * (gx * gy);
+    const unsigned y = r / gx;
+    return make_int3((int)(r - y * gx), (int)y, (int)z);
+}
+
 // Raw buffer loads over one array of `bytes` bytes (< 4 GiB) with 32-bit byte
 // offsets.  The buffer unit range-checks each access: an offset at or past
 // the end returns 0 instead of touching memory.
@@ -194,7 +206,8 @@ enum Slot {
     SLOT_TMP_PROJ2 = 7,
     SLOT_FFT = 8,      // twiddles
     SLOT_MISC = 9,
-    SLOT_BPTAB = 10    // per-view tables of the fast voxel-driven backprojector
+    SLOT_BPTAB = 10,   // per-view tables of the fast voxel-driven backprojector
+    SLOT_ZQVOL = 11    // the sweep's z-quad-layout volume
 };
 
 ct_status fail(ct_context *ctx, ct_status s, const std::string &msg);
@@ -220,6 +233,17 @@ ct_status launch_randn(ct_context *ctx, void *p, ct_dtype dt, int64_t n, uint64_
 ct_status reduce_dot(ct_context *ctx, const void *a, const void *b, int64_t n, ct_dtype dt, int mode, double *out);
 ct_status reduce_tv(ct_context *ctx, const void *f, ct_dtype dt, int M, int N, int Z, int mode, double eps, double *out);
 ct_status launch_convert(ct_context *ctx, const void *in, ct_dtype idt, void *out, ct_dtype odt, int64_t n);
+// [z][y][x] <-> z-quad layout [z/4][y][x][z%4] (z padded to a multiple of 4
+// with zeros); dtype-preserving copies
+ct_status launch_zq_pack(ct_context *ctx, const void *vol, ct_dtype dt, void *volq, int M, int N, int Z);
+ct_status launch_zq_unpack(ct_context *ctx, const void *volq, ct_dtype dt, void *vol, int M, int N, int Z);
+// z-quad FP64 accumulator -> [z][y][x] volume of type dt
+ct_status launch_zq_unpack_f64(ct_context *ctx, const double *accq, void *vol, ct_dtype dt, int M, int N, int Z);
+bool sweep_zq_eligible(const ct_context *ctx, const ct_geometry &g, ct_dtype vdt, int subset);
+ct_status launch_correction_zq(ct_context *ctx, const ct_geometry &g, const void *volq, ct_dtype vdt, float *corr,
+                               const void *meas, ct_dtype mdt, int32_t first, int32_t stride, int32_t nv);
+ct_status launch_voxel_bp_zq(ct_context *ctx, const ct_geometry &g, void *volq, ct_dtype vdt, const void *corr,
+                             int32_t first, int32_t stride, int32_t nv, double beta);
 ct_status launch_forward_mv(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
                             ct_dtype odt, const void *meas, ct_dtype mdt, int32_t first, int32_t stride,
                             int32_t nv, int cmd, int32_t mv0);

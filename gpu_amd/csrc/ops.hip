@@ -73,6 +73,70 @@ ct_status launch_convert(ct_context *ctx, const void *in, ct_dtype idt, void *ou
     return fail(ctx, CT_ERR_DTYPE, "convert: bad dtype");
 }
 
+// z-quad layout conversion: one thread per (x, y, quad); the four z values of
+// a quad are read at stride M*N (coalesced over x) and written contiguously.
+template <typename S, bool PACK>
+__global__ __launch_bounds__(256) void k_zq(const S *__restrict__ in, S *__restrict__ out, int M, int N, int Z) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, q = blockIdx.z;
+    if (x >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t lin = (int64_t)y * M + x;
+    const int64_t qb = ((int64_t)q * MN + lin) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int z = 4 * q + k;
+        if constexpr (PACK) out[qb + k] = z < Z ? in[(int64_t)z * MN + lin] : S(0.0f);
+        else if (z < Z) out[(int64_t)z * MN + lin] = in[qb + k];
+    }
+}
+template <bool PACK>
+static ct_status zq_copy(ct_context *ctx, const void *in, ct_dtype dt, void *out, int M, int N, int Z) {
+    dim3 g((M + 255) / 256, N, (Z + 3) / 4), b(256);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL((k_zq<float, PACK>), g, b, 0, ctx->stream, (const float *)in, (float *)out, M, N, Z); break;
+    case CT_F16: hipLaunchKernelGGL((k_zq<__half, PACK>), g, b, 0, ctx->stream, (const __half *)in, (__half *)out, M, N, Z); break;
+    case CT_BF16:
+        hipLaunchKernelGGL((k_zq<__hip_bfloat16, PACK>), g, b, 0, ctx->stream, (const __hip_bfloat16 *)in,
+                           (__hip_bfloat16 *)out, M, N, Z);
+        break;
+    default: return fail(ctx, CT_ERR_DTYPE, "z-quad copy: unsupported dtype");
+    }
+    return launch_check(ctx, "z-quad copy");
+}
+ct_status launch_zq_pack(ct_context *ctx, const void *vol, ct_dtype dt, void *volq, int M, int N, int Z) {
+    return zq_copy<true>(ctx, vol, dt, volq, M, N, Z);
+}
+ct_status launch_zq_unpack(ct_context *ctx, const void *volq, ct_dtype dt, void *vol, int M, int N, int Z) {
+    return zq_copy<false>(ctx, volq, dt, vol, M, N, Z);
+}
+
+template <typename SO>
+__global__ __launch_bounds__(256) void k_zq_unpack_f64(const double *__restrict__ in, SO *__restrict__ out, int M, int N,
+                                                       int Z) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, q = blockIdx.z;
+    if (x >= M) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t lin = (int64_t)y * M + x;
+    const int64_t qb = ((int64_t)q * MN + lin) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int z = 4 * q + k;
+        if (z < Z) St<SO>::st(out, (int64_t)z * MN + lin, (typename LdT<SO>::type)in[qb + k]);
+    }
+}
+ct_status launch_zq_unpack_f64(ct_context *ctx, const double *accq, void *vol, ct_dtype dt, int M, int N, int Z) {
+    dim3 g((M + 255) / 256, N, (Z + 3) / 4), b(256);
+    switch (dt) {
+    case CT_F32: hipLaunchKernelGGL((k_zq_unpack_f64<float>), g, b, 0, ctx->stream, accq, (float *)vol, M, N, Z); break;
+    case CT_F16: hipLaunchKernelGGL((k_zq_unpack_f64<__half>), g, b, 0, ctx->stream, accq, (__half *)vol, M, N, Z); break;
+    case CT_BF16:
+        hipLaunchKernelGGL((k_zq_unpack_f64<__hip_bfloat16>), g, b, 0, ctx->stream, accq, (__hip_bfloat16 *)vol, M, N, Z);
+        break;
+    case CT_F64: hipLaunchKernelGGL((k_zq_unpack_f64<double>), g, b, 0, ctx->stream, accq, (double *)vol, M, N, Z); break;
+    }
+    return launch_check(ctx, "z-quad unpack");
+}
+
 // y = a*x + b*y
 template <typename S>
 __global__ void k_axpby(const S *x, S *y, int64_t n, float a, float b) {

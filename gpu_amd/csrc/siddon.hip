@@ -295,20 +295,28 @@ __device__ __forceinline__ T siddon_sum(const DGeo<T> &d, T s, T c, int col, int
 //  * one linear voxel offset (in units of ES bytes) with per-axis strides
 //    replaces the three voxel coordinates; the first voxel is bounds-checked
 //    once (a ray entering outside the grid is empty, as in the reference),
-//    and the last segment is clipped at alpha_max instead of bounds-checking
-//    every step.  A step that rounding leaves between the exit plane and
-//    alpha_max can carry an offset past the grid: the forward reads 0 there
-//    and the adjoint scatters nothing, so the pair stays matched;
+//    and the walk ends at alpha_max, clipped to the grid's exit planes as the
+//    walk itself computes them, instead of bounds-checking every step (steps
+//    past the end have length 0; the forward still reads them in bounds or
+//    gets 0 from the range-checked load, the adjoint skips them);
 //  * min/min3 are issued directly (no per-use NaN canonicalisation);
+//  * steps carry alpha lengths; the forward multiplies its sum by dist once;
 //  * the ray length is the closed form dist*(alpha_max - alpha_min), the
 //    telescoped sum of the segments without their rounding.
 // The forward (siddon_walk_fast) and the matched adjoint (k_adjoint) both
 // step through this walker, so their segments are the same numbers.
-template <int ES> struct FastWalk {
+//
+// ZQ: the volume is in the z-quad layout of the SART sweep,
+// [z/4][y][x][z%4] (four z-neighbours share 16 bytes, so the 64 rays of a
+// wave - a few voxels apart in the lateral and z directions - hit a few cache
+// lines per gather instead of one each).  An x or y step moves the offset by
+// a constant; a z step moves it by +-1 inside a quad and by +-(4MN - 3)
+// across a quad boundary, i.e. when the plane crossed has an index that is a
+// multiple of 4.
+template <int ES, bool ZQ = false> struct FastWalk {
     float kx, ky, kz, bx0, by0, bz0, sgxf, sgyf, sgzf, nfx, nfy, nfz, ax, ay, az, ac, amax, dist, len;
-    int dix, diy, diz;
+    int dix, diy, diz, dizb;
     unsigned idx;
-    bool alive;
     // returns false for an empty ray (the forward yields 0 for it)
     __device__ __forceinline__ bool init(const DGeo<float> &d, float s, float c, int col, int row, int src) {
         RayState<float> r = ray_setup(d, s, c, col, row, src);
@@ -318,56 +326,82 @@ template <int ES> struct FastWalk {
         bx0 = (d.bx - r.x1x) * r.ixd; by0 = (d.by - r.x1y) * r.iyd; bz0 = (d.bz - r.x1z) * r.izd;
         sgxf = (float)r.sgx; sgyf = (float)r.sgy; sgzf = (float)r.sgz;
         nfx = (float)r.nx; nfy = (float)r.ny; nfz = (float)r.nz;
-        dix = r.sgx * ES; diy = r.sgy * (int)M * ES; diz = r.sgz * (int)MN * ES;
-        idx = ((unsigned)r.vk * MN + (unsigned)r.vj * M + (unsigned)r.vi) * (unsigned)ES;
-        ax = r.ax; ay = r.ay; az = r.az; ac = r.ac; amax = r.amax; dist = r.dist;
+        if constexpr (ZQ) {
+            dix = r.sgx * 4 * ES; diy = r.sgy * (int)M * 4 * ES;
+            diz = r.sgz * ES; dizb = r.sgz * (4 * (int)MN - 3) * ES;
+            idx = (((unsigned)(r.vk >> 2) * MN + (unsigned)r.vj * M + (unsigned)r.vi) * 4u + ((unsigned)r.vk & 3u)) *
+                  (unsigned)ES;
+        } else {
+            dix = r.sgx * ES; diy = r.sgy * (int)M * ES; diz = r.sgz * (int)MN * ES; dizb = diz;
+            idx = ((unsigned)r.vk * MN + (unsigned)r.vj * M + (unsigned)r.vi) * (unsigned)ES;
+        }
+        ax = r.ax; ay = r.ay; az = r.az; ac = r.ac; dist = r.dist;
+        // end the walk at the grid's exit planes as this walk computes them,
+        // so no step past the grid carries a nonzero length
+        amax = r.amax;
+        if (r.ax != __builtin_inff()) {
+            const int n = r.sgx > 0 ? (int)M : 0;
+            amax = hmin(amax, n == r.nx ? r.ax : fmaf((float)n, kx, bx0));
+        }
+        if (r.ay != __builtin_inff()) {
+            const int n = r.sgy > 0 ? (int)N : 0;
+            amax = hmin(amax, n == r.ny ? r.ay : fmaf((float)n, ky, by0));
+        }
+        if (r.az != __builtin_inff()) {
+            const int n = r.sgz > 0 ? (int)Z : 0;
+            amax = hmin(amax, n == r.nz ? r.az : fmaf((float)n, kz, bz0));
+        }
         len = dist * (amax - ac);
-        alive = true;
         return true;
     }
-    // one step: the segment length (0 once the ray has ended) and the offset
-    // of the voxel it lies in
+    // one step: the segment's alpha length (times dist = its length in mm)
+    // and the offset of the voxel it lies in.  alpha_min <= every pending
+    // crossing, so once ac reaches alpha_max every later step has length 0;
+    // the caller tests for the end of the ray once per batch (ended()).
     __device__ __forceinline__ float step(unsigned &id) {
-        const bool ok = alive && (amax - ac > loop_eps<float>());
         const float m = hmin(hmin3(ax, ay, az), amax);
-        const float seg = ok ? dist * (m - ac) : 0.0f;
+        const float dl = m - ac;
         id = idx;
-        const bool sx = ok && (ax == m), sy = ok && (ay == m), sz = ok && (az == m);
+        const bool sx = ax == m, sy = ay == m, sz = az == m;
+        // ZQ: the z plane about to be crossed bounds a quad
+        const bool qb = ZQ && __builtin_amdgcn_fractf(nfz * 0.25f) == 0.0f;
         const float nx1 = nfx + sgxf, ny1 = nfy + sgyf, nz1 = nfz + sgzf;
         const float ax1 = fmaf(nx1, kx, bx0), ay1 = fmaf(ny1, ky, by0), az1 = fmaf(nz1, kz, bz0);
         nfx = sx ? nx1 : nfx; ax = sx ? ax1 : ax;
         nfy = sy ? ny1 : nfy; ay = sy ? ay1 : ay;
         nfz = sz ? nz1 : nfz; az = sz ? az1 : az;
-        idx += (unsigned)((sx ? dix : 0) + (sy ? diy : 0) + (sz ? diz : 0));
-        ac = ok ? m : ac;
-        alive = ok;
-        return seg;
+        idx += (unsigned)((sx ? dix : 0) + (sy ? diy : 0) + (sz ? (qb ? dizb : diz) : 0));
+        ac = m;
+        return dl;
     }
+    __device__ __forceinline__ bool ended() const { return !(amax - ac > loop_eps<float>()); }
 };
 
-constexpr int FAST_B = 8;   // steps per batch: their gathers are in flight together
+constexpr int FAST_B = 16;  // steps per batch: their gathers are in flight together
 
 // Fast-mode ray sum: batches of FAST_B steps whose loads are issued together;
 // each batch is summed in FP32 and the batch sums accumulate in FP64.
 // Volumes under 4 GiB are read with range-checked buffer loads on 32-bit
 // byte offsets; larger ones use 64-bit addresses with an explicit range test.
-template <typename SV, bool BUF>
+template <typename SV, bool BUF, bool ZQ = false>
 __device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s, float c, int col, int row, int src,
                                                   const SV *__restrict__ f, float &len_out) {
     constexpr int ES = BUF ? (int)sizeof(SV) : 1;
-    FastWalk<ES> w;
+    FastWalk<ES, ZQ> w;
     len_out = 0.0f;
     if (!w.init(d, s, c, col, row, src)) return 0.0f;
-    const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (unsigned)d.Z;
+    // elements of the array (the z-quad layout pads z to a multiple of 4)
+    const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (ZQ ? (unsigned)((d.Z + 3) & ~3) : (unsigned)d.Z);
     // (for the 64-bit path the resource is built but never used)
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(f, BUF ? nvox * (unsigned)sizeof(SV) : 0u);
+    // sum of alpha-length * value; times dist at the end
     double sum = 0.0;
     int guard = d.M + d.N + d.Z + 4;
-    while (w.alive && guard > 0) {
+    do {
         unsigned ids[FAST_B];
-        float seg[FAST_B];
+        float dl[FAST_B];
 #pragma unroll
-        for (int u = 0; u < FAST_B; ++u) seg[u] = w.step(ids[u]);
+        for (int u = 0; u < FAST_B; ++u) dl[u] = w.step(ids[u]);
         guard -= FAST_B;
         float fv[FAST_B];
 #pragma unroll
@@ -375,13 +409,13 @@ __device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s,
             if constexpr (BUF) fv[u] = bload<SV>(rs, ids[u]);
             else fv[u] = ids[u] < nvox ? (float)ldf(f, (int64_t)ids[u]) : 0.0f;
         }
-        float ps = seg[0] * fv[0];
+        float ps = dl[0] * fv[0];
 #pragma unroll
-        for (int u = 1; u < FAST_B; ++u) ps = fmaf(seg[u], fv[u], ps);
+        for (int u = 1; u < FAST_B; ++u) ps = fmaf(dl[u], fv[u], ps);
         sum += (double)ps;
-    }
+    } while (!w.ended() && guard > 0);
     len_out = w.len;
-    return (float)sum;
+    return (float)(sum * (double)w.dist);
 }
 
 template <typename SV>
@@ -392,15 +426,19 @@ __device__ __forceinline__ float siddon_sum_fast(const DGeo<float> &d, float s, 
     return siddon_walk_fast<SV, false>(d, s, c, col, row, src, f, len_out);
 }
 
-__device__ __forceinline__ void tile_coords(int &col, int &row) {
+// 16x16-pixel tile per workgroup (XCD-aware tile order), 8x8 per wave;
+// returns the tile's image slot
+__device__ __forceinline__ int tile_coords(int &col, int &row) {
     const int tid = threadIdx.x;
     const int w = tid >> 6, l = tid & 63;
-    col = blockIdx.x * 16 + (w & 1) * 8 + (l & 7);
-    row = blockIdx.y * 16 + (w >> 1) * 8 + (l >> 3);
+    const int3 t = xcd_tile();
+    col = t.x * 16 + (w & 1) * 8 + (l & 7);
+    row = t.y * 16 + (w >> 1) * 8 + (l >> 3);
+    return t.z;
 }
 
 // CMD 0: ray sum; CMD 1: SART correction (b - sum)/len
-// (V63/kernel_IterativeRecon_universal.cu:554-563).  blockIdx.z = q*NS + src
+// (V63/kernel_IterativeRecon_universal.cu:554-563).  image slot = q*NS + src
 // for batch slot q; view = first + q*stride; meas is indexed by view - mv0.
 template <typename T, typename SV, typename SO, typename SM, int CMD, bool COMP>
 __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict__ ang,
@@ -408,9 +446,9 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
                                                  const SM *__restrict__ meas, int first, int stride,
                                                  int mv0, int z0) {
     int col, row;
-    tile_coords(col, row);
+    const int slot = tile_coords(col, row);
     if (col >= d.R || row >= d.ZP) return;
-    const int img = blockIdx.z + z0;
+    const int img = slot + z0;
     const int q = img / d.NS, src = img - q * d.NS;
     const int v = first + q * stride;
     const T s = ang[2 * v], c = ang[2 * v + 1];
@@ -430,6 +468,33 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
     St<SO>::st(out, (int64_t)img * P + pix, r);
 }
 
+// SART correction (CMD 1) of the sweep on a z-quad-layout volume (fast
+// mode, FP32 arithmetic, volume under 4 GiB): the same rays, segments and
+// output as k_forward<float, SV, float, SM, 1, true>.
+template <typename SV, typename SM>
+__global__ __launch_bounds__(256) void k_correction_zq(DGeo<float> d, const float *__restrict__ ang,
+                                                       const SV *__restrict__ fq, float *__restrict__ out,
+                                                       const SM *__restrict__ meas, int first, int stride, int mv0,
+                                                       int z0) {
+    int col, row;
+    const int slot = tile_coords(col, row);
+    if (col >= d.R || row >= d.ZP) return;
+    const int img = slot + z0;
+    const int q = img / d.NS, src = img - q * d.NS;
+    const int v = first + q * stride;
+    const float s = ang[2 * v], c = ang[2 * v + 1];
+    float len;
+    const float sum = siddon_walk_fast<SV, true, true>(d, s, c, col, row, src, fq, len);
+    const int64_t P = (int64_t)d.R * d.ZP;
+    const int64_t pix = (int64_t)row * d.R + col;
+    float r = 0.0f;
+    if (!(len < d.vz * 1e-6f)) {
+        const float b = (float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix);
+        r = (b - sum) / len;
+    }
+    out[(int64_t)img * P + pix] = r;
+}
+
 // Data fidelity sum (Ax - b)^2 fused into the ray traversal: Ax never goes to
 // memory.  Each 256-ray block writes one FP64 partial (wave64 shuffles, then
 // LDS across the 4 waves); the partials are folded in fixed order afterwards.
@@ -440,8 +505,8 @@ __global__ __launch_bounds__(256) void k_residual(DGeo<T> d, const T *__restrict
                                                   const SM *__restrict__ meas, int view0, int z0,
                                                   double *__restrict__ part) {
     int col, row;
-    tile_coords(col, row);
-    const int img = blockIdx.z + z0;
+    const int slot = tile_coords(col, row);
+    const int img = slot + z0;
     double e2 = 0.0;
     if (col < d.R && row < d.ZP) {
         const int q = img / d.NS, src = img - q * d.NS;
@@ -461,8 +526,10 @@ __global__ __launch_bounds__(256) void k_residual(DGeo<T> d, const T *__restrict
     __shared__ double wsum[4];
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
     __syncthreads();
+    // one slot per workgroup: launches cover images [z0, z0 + gridDim.z)
     if (threadIdx.x == 0)
-        part[((int64_t)img * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+        part[((int64_t)z0 * gridDim.y + blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+            (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 template <typename T, typename SV, typename SM>
@@ -529,14 +596,14 @@ template <> __device__ __forceinline__ void atomic_add<double>(double *p, double
 // closed-form length, and no scatter for an offset past the grid (the
 // forward reads 0 there).  Otherwise a first pass measures the ray length
 // of the reference traversal.
-template <typename T, typename SP, typename A, bool FAST>
+template <typename T, typename SP, typename A, bool FAST, bool ZQ = false>
 __global__ __launch_bounds__(256) void k_adjoint(DGeo<T> d, const T *__restrict__ ang,
                                                  const SP *__restrict__ proj, A *__restrict__ acc,
                                                  int view0, int z0) {
     int col, row;
-    tile_coords(col, row);
+    const int slot = tile_coords(col, row);
     if (col >= d.R || row >= d.ZP) return;
-    const int img = blockIdx.z + z0;
+    const int img = slot + z0;
     const int q = img / d.NS, src = img - q * d.NS;
     const int v = view0 + q;
     const int64_t P = (int64_t)d.R * d.ZP;
@@ -544,20 +611,20 @@ __global__ __launch_bounds__(256) void k_adjoint(DGeo<T> d, const T *__restrict_
     if (y == (T)0) return;
     const T s = ang[2 * v], c = ang[2 * v + 1];
     if constexpr (FAST) {
-        FastWalk<1> w;
+        FastWalk<1, ZQ> w;
         if (!w.init(d, s, c, col, row, src)) return;
         if (w.len < d.vz * 1e-6f) return;
-        const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (unsigned)d.Z;
+        const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (ZQ ? (unsigned)((d.Z + 3) & ~3) : (unsigned)d.Z);
         int guard = d.M + d.N + d.Z + 4;
-        while (w.alive && guard > 0) {
+        do {
 #pragma unroll
             for (int u = 0; u < FAST_B; ++u) {
                 unsigned id;
-                const float seg = w.step(id);
-                if (seg != 0.0f && id < nvox) atomic_add<A>(acc + id, (A)seg * (A)y);
+                const float dl = w.step(id);
+                if (dl != 0.0f && id < nvox) atomic_add<A>(acc + id, (A)(w.dist * dl) * (A)y);
             }
             guard -= FAST_B;
-        }
+        } while (!w.ended() && guard > 0);
     } else {
         const T len = siddon_traverse(d, s, c, col, row, src, [](int, T) {});
         if (len < d.vz * (T)1e-6) return;
@@ -642,6 +709,50 @@ ct_status launch_forward_mv(ct_context *ctx, const ct_geometry &g, const void *v
     return fail(ctx, CT_ERR_DTYPE, "unsupported volume dtype");
 }
 
+template <typename SV, typename SM>
+static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const void *volq, float *corr,
+                                   const void *meas, int first, int stride, int nv) {
+    ct_status st;
+    const float *ang = (const float *)angle_table(ctx, g, false, false, &st);
+    if (!ang) return st;
+    DGeo<float> d = derive<float>(g);
+    const int nimg = nv * d.NS;
+    ProfScope ps(ctx, CT_K_FORWARD);
+    for (int z0 = 0; z0 < nimg; z0 += 65535) {
+        const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
+        dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
+        hipLaunchKernelGGL((k_correction_zq<SV, SM>), grid, dim3(256), 0, ctx->stream, d, ang, (const SV *)volq, corr,
+                           (const SM *)meas, first, stride, 0, z0);
+    }
+    return launch_check(ctx, "correction");
+}
+
+template <typename SV>
+static ct_status corr_zq_meas(ct_context *ctx, const ct_geometry &g, const void *volq, float *corr, const void *meas,
+                              ct_dtype mdt, int first, int stride, int nv) {
+    switch (mdt) {
+    case CT_F32: return run_correction_zq<SV, float>(ctx, g, volq, corr, meas, first, stride, nv);
+    case CT_F16: return run_correction_zq<SV, __half>(ctx, g, volq, corr, meas, first, stride, nv);
+    case CT_BF16: return run_correction_zq<SV, __hip_bfloat16>(ctx, g, volq, corr, meas, first, stride, nv);
+    case CT_F64: return run_correction_zq<SV, double>(ctx, g, volq, corr, meas, first, stride, nv);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "unsupported measurement dtype");
+}
+
+// The sweep's SART correction on a z-quad-layout volume (views first,
+// first + stride, ...; the measurement stack is indexed by view).
+ct_status launch_correction_zq(ct_context *ctx, const ct_geometry &g, const void *volq, ct_dtype vdt, float *corr,
+                               const void *meas, ct_dtype mdt, int32_t first, int32_t stride, int32_t nv) {
+    if (nv <= 0) return CT_OK;
+    switch (vdt) {
+    case CT_F32: return corr_zq_meas<float>(ctx, g, volq, corr, meas, mdt, first, stride, nv);
+    case CT_F16: return corr_zq_meas<__half>(ctx, g, volq, corr, meas, mdt, first, stride, nv);
+    case CT_BF16: return corr_zq_meas<__hip_bfloat16>(ctx, g, volq, corr, meas, mdt, first, stride, nv);
+    default: break;
+    }
+    return fail(ctx, CT_ERR_DTYPE, "z-quad correction: unsupported volume dtype");
+}
+
 ct_status launch_forward(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
                          ct_dtype odt, const void *meas, ct_dtype mdt, int32_t first, int32_t stride,
                          int32_t nv, int cmd) {
@@ -660,10 +771,12 @@ static ct_status run_adjoint(ct_context *ctx, const ct_geometry &g, const void *
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
-        // the fast forward's walk in fast mode, so the pair stays matched
+        // the fast forward's walk in fast mode, so the pair stays matched; the
+        // accumulator is then in the z-quad layout (neighbouring rays' atomics
+        // share cache lines)
         if (std::is_same<T, float>::value && ctx->math == CT_MATH_FAST)
-            hipLaunchKernelGGL((k_adjoint<T, SP, A, std::is_same<T, float>::value>), grid, block, 0, ctx->stream, d,
-                               ang, (const SP *)proj, acc, view0, z0);
+            hipLaunchKernelGGL((k_adjoint<T, SP, A, std::is_same<T, float>::value, true>), grid, block, 0,
+                               ctx->stream, d, ang, (const SP *)proj, acc, view0, z0);
         else
             hipLaunchKernelGGL((k_adjoint<T, SP, A, false>), grid, block, 0, ctx->stream, d, ang, (const SP *)proj,
                                acc, view0, z0);
@@ -683,11 +796,14 @@ ct_status launch_adjoint(ct_context *ctx, const ct_geometry &g, const void *proj
     if (pdt == CT_F64) return fail(ctx, CT_ERR_DTYPE, "F64 projections need an F64 volume");
     // FP32 segments scattered into an FP64 accumulator (f64 atomics): the
     // ~10^2 contributions per voxel are summed without FP32 rounding, then
-    // rounded once to the output type
+    // rounded once to the output type.  Fast mode accumulates in the z-quad
+    // layout and unpacks while converting.
+    const bool zq = ctx->math == CT_MATH_FAST;
+    const int64_t Va = zq ? (int64_t)g.nx * g.ny * ((g.nz + 3) & ~3) : V;
     ct_status st;
-    double *acc = (double *)slot(ctx, SLOT_TMP_VOL, sizeof(double) * V, &st);
+    double *acc = (double *)slot(ctx, SLOT_TMP_VOL, sizeof(double) * Va, &st);
     if (!acc) return st;
-    st = launch_fill(ctx, acc, CT_F64, V, 0.0);
+    st = launch_fill(ctx, acc, CT_F64, Va, 0.0);
     if (st != CT_OK) return st;
     switch (pdt) {
     case CT_F32: st = run_adjoint<float, float, double>(ctx, g, proj, acc, view0, nv); break;
@@ -696,6 +812,7 @@ ct_status launch_adjoint(ct_context *ctx, const ct_geometry &g, const void *proj
     default: return fail(ctx, CT_ERR_DTYPE, "unsupported projection dtype");
     }
     if (st != CT_OK) return st;
+    if (zq) return launch_zq_unpack_f64(ctx, acc, vol, vdt, g.nx, g.ny, g.nz);
     return launch_convert(ctx, acc, CT_F64, vol, vdt, V);
 }
 

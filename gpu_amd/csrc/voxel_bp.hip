@@ -292,8 +292,91 @@ __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__r
     if (i == 0) rowT[(int64_t)img * d.ZP + j] = make_float2(qz, 1.0f / ez);
 }
 
+constexpr int BPX = 16, BPY = 16;
+// z-run per lane: 16 voxels for single-image launches, 8 when per-voxel sums
+// are carried across images (they live in registers)
+template <bool ONE> constexpr int zrun() { return ONE ? 16 : 8; }
+constexpr int LW = 64;          // staged footprint columns
+constexpr int LP = LW + 8;      // LDS row pitch: 8 zero columns for the NC overrun
+constexpr int LH = 48;          // staged footprint rows
+constexpr int NCMAX = 8;
+
+// Packed-half chord engine for reduced-precision storage.  Two detector
+// columns share one 32-bit register (lo half, hi half).  Alpha values are
+// taken in voxel-local units, l = (alpha - alpha0)*kl with
+// alpha0 = (t_centre - DOD)/(DSO - DOD) (alpha is linear in the depth t along
+// every ray) and kl = (DSO - DOD)/|voxel diagonal|, so the chord interval of
+// a ray through the voxel lies within |l| <= 1/2 and FP16 resolves it to
+// ~5e-4 of the voxel diagonal; values far outside only take part in the
+// min/max and may round or saturate freely.  Products are accumulated in
+// FP32 by v_dot2_f32_f16; W and L come out in local units (times kl), so
+// W/L is unchanged and both are scaled back before the update.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pk_rtz(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+__device__ __forceinline__ unsigned pk_rne(float a, float b) {
+    const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)a);
+    const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)b);
+    return lo | (hi << 16);
+}
+__device__ __forceinline__ unsigned pk_min(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_pk_min_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ unsigned pk_max(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_pk_max_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// min / max of the two halves of a, in both halves
+__device__ __forceinline__ unsigned pk_min_bc(unsigned a) {
+    unsigned r;
+    asm("v_pk_min_f16 %0, %1, %1 op_sel:[0,1] op_sel_hi:[1,0]" : "=v"(r) : "v"(a));
+    return r;
+}
+__device__ __forceinline__ unsigned pk_max_bc(unsigned a) {
+    unsigned r;
+    asm("v_pk_max_f16 %0, %1, %1 op_sel:[0,1] op_sel_hi:[1,0]" : "=v"(r) : "v"(a));
+    return r;
+}
+// max(a - b, 0) per half
+__device__ __forceinline__ unsigned pk_span(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_pk_add_f16 %0, %1, %2 neg_lo:[0,1] neg_hi:[0,1]\n\tv_pk_max_f16 %0, %0, 0" : "=&v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float dot2(unsigned a, unsigned b, float c) {
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c, false);
+}
+
+// rows of the packed-half engine: per row (A, B) = (kl/ez, kl*qz/ez) from
+// LDS, per column pair the staged (c*|S-P|, |S-P|) half pairs
+template <int NP>
+__device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
+                                          int cbase, float zl, float zh, float nA, const unsigned *lo2,
+                                          const unsigned *hi2, float &wk, float &lk) {
+    for (int j = z0; j <= z1; ++j) {
+        const float2 rt = sr[j];
+        const uint2 *prow = sp + j * LP + cbase;
+        uint2 pv[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) pv[p] = prow[2 * p];
+        const float t = nA - rt.y;
+        const unsigned az = pk_rtz(fmaf(zl, rt.x, t), fmaf(zh, rt.x, t));
+        const unsigned loz = pk_min_bc(az), hiz = pk_max_bc(az);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const unsigned span = pk_span(pk_min(hi2[p], hiz), pk_max(lo2[p], loz));
+            wk = dot2(span, pv[p].x, wk);
+            lk = dot2(span, pv[p].y, lk);
+        }
+    }
+}
+
 // LDS-staged fast backprojector.  A workgroup owns a 16x16 (x, y) tile of
-// ZR voxels in z; each lane owns the ZR voxels of one (x, y).  Per image the
+// ZR (16 or 8) voxels in z; each lane owns the ZR voxels of one (x, y).  Per image the
 // workgroup stages its detector footprint - the projection of the tile's box,
 // widened by one pixel each way - from the per-view tables into LDS: pixel
 // values (c*|S-P|, |S-P|) in rows of LP entries (columns past the footprint
@@ -307,11 +390,6 @@ __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__r
 // cut per pixel (they move W and L by less than 1e-6*vx each); the update
 // threshold on L stays.  Footprints larger than the LDS window take the
 // per-voxel table path for that image.
-constexpr int BPX = 16, BPY = 16, ZR = 8;
-constexpr int LW = 80;          // staged footprint columns
-constexpr int LP = LW + 8;      // LDS row pitch: 8 zero columns for the NC overrun
-constexpr int LH = 36;          // staged footprint rows
-constexpr int NCMAX = 8;
 
 template <int NC>
 __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
@@ -350,20 +428,31 @@ __device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__
 
 // ONE: the launch backprojects a single image (one view, one source), so each
 // voxel is updated as soon as its sums are formed and no per-voxel sums are
-// carried across images.
-template <typename SV, bool ONE>
+// carried across images.  H: the packed-half chord engine (reduced-precision
+// storage paths).  ZQ: the volume is in the sweep's z-quad layout
+// [z/4][y][x][z%4].
+template <bool ZQ>
+__device__ __forceinline__ int64_t vox_index(const DGeo<float> &d, int ix, int iy, int iz) {
+    if constexpr (ZQ) return (((int64_t)(iz >> 2) * d.N + iy) * d.M + ix) * 4 + (iz & 3);
+    else return ((int64_t)iz * d.N + iy) * d.M + ix;
+}
+
+template <typename SV, bool ONE, bool H, bool ZQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void k_voxel_bp_lds(DGeo<float> d, const float *__restrict__ ang,
                                                       SV *__restrict__ f, const float2 *__restrict__ pixT,
                                                       const float4 *__restrict__ colT,
                                                       const float2 *__restrict__ rowT, int first, int stride,
                                                       int nv, float beta, int mode) {
-    __shared__ float2 sp[LH * LP];
+    __shared__ float2 sp[LH * LP];   // (c*|S-P|, |S-P|); H: column-pair halves
     __shared__ float4 sc[LP];
-    __shared__ float2 sr[LH];
+    __shared__ float2 sr[LH];        // (qz, 1/ez); H: (kl/ez, kl*qz/ez)
+    uint2 *const sph = reinterpret_cast<uint2 *>(sp);
     const int tid = threadIdx.y * BPX + threadIdx.x;
-    const int ix = blockIdx.x * BPX + threadIdx.x;
-    const int iy = blockIdx.y * BPY + threadIdx.y;
-    const int iz0 = blockIdx.z * ZR;
+    const int3 tile = xcd_tile();   // each XCD: a contiguous z range of tiles
+    const int ix = tile.x * BPX + threadIdx.x;
+    const int iy = tile.y * BPY + threadIdx.y;
+    constexpr int ZR = zrun<ONE>();
+    const int iz0 = tile.z * ZR;
     const int nz = min(ZR, d.Z - iz0);
     bool active = ix < d.M && iy < d.N;
     if (active) {   // cylinder mask
@@ -384,10 +473,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
     const float hvx = 0.5f * d.vx, hvy = 0.5f * d.vy, hvz = 0.5f * d.vz;
     const float xl = cx - hvx, xh = cx + hvx, yl = cy - hvy, yh = cy + hvy;
     // tile box (clipped to the volume) for the footprint
-    const int tx0 = blockIdx.x * BPX, ty0 = blockIdx.y * BPY;
+    const int tx0 = tile.x * BPX, ty0 = tile.y * BPY;
     const float TX0 = d.bx + (float)tx0 * d.vx, TX1 = d.bx + (float)min(tx0 + BPX, d.M) * d.vx;
     const float TY0 = d.by + (float)ty0 * d.vy, TY1 = d.by + (float)min(ty0 + BPY, d.N) * d.vy;
     const float TZ0 = d.bz + (float)iz0 * d.vz, TZ1 = d.bz + (float)(iz0 + nz) * d.vz;
+    // packed-half engine: alpha scale kl and its inverse
+    const float vdiag = sqrtf(d.vx * d.vx + d.vy * d.vy + d.vz * d.vz);
+    const float kl = (d.dso - d.dod) / vdiag, ikl = vdiag / (d.dso - d.dod);
     float W[ONE ? 1 : ZR], L[ONE ? 1 : ZR];
 #pragma unroll
     for (int k = 0; k < (ONE ? 1 : ZR); ++k) { W[k] = 0.0f; L[k] = 0.0f; }
@@ -449,12 +541,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             float rk = 0.0f;
             if (d.tb) rk = (d.dso - d.dod) * frcp(cx * c + cy * s - d.dod);
             __syncthreads();   // the previous image's readers are done
-            for (int i = tid; i < nrow * LP; i += BPX * BPY) {
-                const int r = i / LP, cl = i - r * LP;
-                sp[i] = cl < ncw ? pimg[(int64_t)(R0 + r) * d.R + C0 + cl] : make_float2(0.0f, 0.0f);
+            {
+                // 4 rows x 64 columns per pass; columns past the footprint
+                // (up to the row pitch) are zero
+                const int cl = tid & 63;
+                for (int r = tid >> 6; r < nrow; r += 4) {
+                    const float2 *g = pimg + (int64_t)(R0 + r) * d.R + C0;
+                    const float2 a = cl < ncw ? g[cl] : make_float2(0.0f, 0.0f);
+                    if constexpr (H) {
+                        const float2 b = cl + 1 < ncw ? g[cl + 1] : make_float2(0.0f, 0.0f);
+                        sph[r * LP + cl] = make_uint2(pk_rne(a.x, b.x), pk_rne(a.y, b.y));
+                    } else {
+                        sp[r * LP + cl] = a;
+                    }
+                    if (cl < LP - LW) {
+                        if constexpr (H) sph[r * LP + LW + cl] = make_uint2(0u, 0u);
+                        else sp[r * LP + LW + cl] = make_float2(0.0f, 0.0f);
+                    }
+                }
             }
             if (tid < LP) sc[tid] = tid < ncw ? cimg[C0 + tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (tid < nrow) sr[tid] = rimg[R0 + tid];
+            if (tid < nrow) {
+                const float2 rt = rimg[R0 + tid];
+                sr[tid] = H ? make_float2(kl * rt.y, kl * rt.x * rt.y) : rt;
+            }
             __syncthreads();
             if (!active || yoff) continue;
             const int y0 = max(y0r, C0), y1 = min(min(y1r, C1), y0 + NCMAX - 1);
@@ -470,12 +580,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             float lo[NCMAX], hi[NCMAX];
 #pragma unroll
             for (int cc = 0; cc < NCMAX; ++cc) {
+                if (H && cc >= NC) { lo[cc] = __builtin_inff(); hi[cc] = -__builtin_inff(); continue; }
                 const float4 ct = sc[min(cbase + cc, LP - 1)];
                 const float ax0 = (xl - ct.x) * ct.z, ax1 = (xh - ct.x) * ct.z;
                 const float ay0 = (yl - ct.y) * ct.w, ay1 = (yh - ct.y) * ct.w;
                 const bool in = cc < ncol;
                 lo[cc] = in ? hmax(hmin(ax0, ax1), hmin(ay0, ay1)) : __builtin_inff();
                 hi[cc] = in ? hmin(hmax(ax0, ax1), hmax(ay0, ay1)) : -__builtin_inff();
+            }
+            // H: the column intervals in voxel-local alpha units, paired
+            const float nA = (d.dod - (cx * c + cy * s)) / vdiag;
+            unsigned lo2[NCMAX / 2], hi2[NCMAX / 2];
+            if constexpr (H) {
+#pragma unroll
+                for (int p = 0; p < NCMAX / 2; ++p) {
+                    if (2 * p >= NC) break;
+                    lo2[p] = pk_rtz(fmaf(lo[2 * p], kl, nA), fmaf(lo[2 * p + 1], kl, nA));
+                    hi2[p] = pk_rtz(fmaf(hi[2 * p], kl, nA), fmaf(hi[2 * p + 1], kl, nA));
+                }
             }
 #pragma unroll 1
             for (int k = 0; k < nz; ++k) {
@@ -498,11 +620,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                 z0 = max(z0, R0) - R0;
                 z1 = min(z1, R1) - R0;
                 float wk = ONE ? 0.0f : W[k], lk = ONE ? 0.0f : L[k];
-                if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                else bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                if constexpr (ONE) bp_update(d, f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, wk, lk, beta, mode);
+                if constexpr (H) {
+                    if (NC == 8) bp_rows_h<4>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
+                    else if (NC >= 5) bp_rows_h<3>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
+                    else bp_rows_h<2>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
+                } else {
+                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                }
+                if constexpr (ONE)
+                    bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,
+                              mode);
                 else { W[k] = wk; L[k] = lk; }
             }
         }
@@ -512,7 +642,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
 #pragma unroll
         for (int k = 0; k < ZR; ++k) {
             if (k >= nz) break;
-            bp_update(d, f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, W[k], L[k], beta, mode);
+            bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? W[k] * ikl : W[k], H ? L[k] * ikl : L[k],
+                      beta, mode);
         }
     }
 }
@@ -588,7 +719,7 @@ __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict
 // (1 + r_max)*D + dr*D*(rho + |off|) in u and
 // (1 + r_max)*H + dr*D*max|z - z_source| in v; the floor(x + 0.5) bounds add
 // one pixel and the footprint pad two more.
-static bool bp_lds_fits(const DGeo<float> &d) {
+static bool bp_lds_fits(const DGeo<float> &d, int zr) {
     const double vx = d.vx, vy = d.vy, vz = d.vz, dso = d.dso, dod = d.dod;
     const double xs[2] = {(double)d.bx, (double)d.bx + d.M * vx}, ys[2] = {(double)d.by, (double)d.by + d.N * vy};
     double rho = 0;
@@ -611,7 +742,44 @@ static bool bp_lds_fits(const DGeo<float> &d) {
     const double lat = rho + std::fabs((double)d.sy);
     auto cols = [&](double D) { return ((1.0 + rmax) * D + dr * D * lat) / (double)d.du; };
     auto rows = [&](double D, double H) { return ((1.0 + rmax) * H + dr * D * zrel) / (double)d.dv; };
-    return cols(DT) + 4.0 <= LW && rows(DT, ZR * vz) + 4.0 <= LH && cols(DV) + 2.0 <= NCMAX;
+    return cols(DT) + 4.0 <= LW && rows(DT, zr * vz) + 4.0 <= LH && cols(DV) + 2.0 <= NCMAX;
+}
+
+// per-view tables, then the LDS-staged update (fast FP32 arithmetic)
+template <typename SV, bool ZQ>
+static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *ang, void *vol, const void *corr,
+                            int first, int stride, int nv, double beta, int mode) {
+    ct_status st;
+    const int nimg = nv * d.NS;
+    const size_t P = (size_t)d.R * d.ZP;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pix_b = up(sizeof(float2) * P * nimg), col_b = up(sizeof(float4) * (size_t)d.R * nv);
+    const size_t row_b = up(sizeof(float2) * (size_t)d.ZP * nimg);
+    char *tab = (char *)slot(ctx, SLOT_BPTAB, pix_b + col_b + row_b, &st);
+    if (!tab) return st;
+    float2 *pix = (float2 *)tab;
+    float4 *colT = (float4 *)(tab + pix_b);
+    float2 *rowT = (float2 *)(tab + pix_b + col_b);
+    ProfScope ps(ctx, CT_K_VOXEL_BP);
+    hipLaunchKernelGGL(k_bp_prep, dim3((d.R + 255) / 256, d.ZP, nimg), dim3(256), 0, ctx->stream, d, ang,
+                       (const float *)corr, first, stride, pix, colT, rowT);
+    const int zr = nimg == 1 ? zrun<true>() : zrun<false>();
+    dim3 lgrid((d.M + BPX - 1) / BPX, (d.N + BPY - 1) / BPY, (d.Z + zr - 1) / zr);
+    // reduced-precision storage takes the packed-half chord engine
+    constexpr bool H = !std::is_same<SV, float>::value;
+    if (nimg == 1)
+        hipLaunchKernelGGL((k_voxel_bp_lds<SV, true, H, ZQ>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
+                           (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
+                           nv, (float)beta, mode);
+    else
+        hipLaunchKernelGGL((k_voxel_bp_lds<SV, false, H, ZQ>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
+                           (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
+                           nv, (float)beta, mode);
+    return launch_check(ctx, "voxel backprojection");
+}
+
+static bool lds_eligible(const ct_context *ctx, const DGeo<float> &d, int nimg) {
+    return ctx->math == CT_MATH_FAST && nimg <= 65535 && bp_lds_fits(d, nimg == 1 ? zrun<true>() : zrun<false>());
 }
 
 template <typename T, typename SV>
@@ -624,32 +792,8 @@ static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const 
     dim3 block(64, 4, 1);
     dim3 grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
     if constexpr (std::is_same<T, float>::value) {
-        const int nimg = nv * d.NS;
-        if (ctx->math == CT_MATH_FAST && nimg <= 65535 && bp_lds_fits(d)) {
-            // per-view tables, then the table-driven update
-            const size_t P = (size_t)d.R * d.ZP;
-            auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-            const size_t pix_b = up(sizeof(float2) * P * nimg), col_b = up(sizeof(float4) * (size_t)d.R * nv);
-            const size_t row_b = up(sizeof(float2) * (size_t)d.ZP * nimg);
-            char *tab = (char *)slot(ctx, SLOT_BPTAB, pix_b + col_b + row_b, &st);
-            if (!tab) return st;
-            float2 *pix = (float2 *)tab;
-            float4 *colT = (float4 *)(tab + pix_b);
-            float2 *rowT = (float2 *)(tab + pix_b + col_b);
-            ProfScope ps(ctx, CT_K_VOXEL_BP);
-            hipLaunchKernelGGL(k_bp_prep, dim3((d.R + 255) / 256, d.ZP, nimg), dim3(256), 0, ctx->stream, d, ang,
-                               (const float *)corr, first, stride, pix, colT, rowT);
-            dim3 lgrid((d.M + BPX - 1) / BPX, (d.N + BPY - 1) / BPY, (d.Z + ZR - 1) / ZR);
-            if (nimg == 1)
-                hipLaunchKernelGGL((k_voxel_bp_lds<SV, true>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
-                                   (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first,
-                                   stride, nv, (float)beta, mode);
-            else
-                hipLaunchKernelGGL((k_voxel_bp_lds<SV, false>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
-                                   (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first,
-                                   stride, nv, (float)beta, mode);
-            return launch_check(ctx, "voxel backprojection");
-        }
+        if (lds_eligible(ctx, d, nv * d.NS))
+            return run_bp_lds<SV, false>(ctx, d, ang, vol, corr, first, stride, nv, beta, mode);
         if (ctx->math == CT_MATH_FAST) {
             ProfScope ps(ctx, CT_K_VOXEL_BP);
             hipLaunchKernelGGL((k_voxel_bp<T, SV, true>), grid, block, 0, ctx->stream, d, ang, (SV *)vol,
@@ -661,6 +805,36 @@ static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const 
     hipLaunchKernelGGL((k_voxel_bp<T, SV, false>), grid, block, 0, ctx->stream, d, ang, (SV *)vol, (const T *)corr,
                        first, stride, nv, (T)beta, mode);
     return launch_check(ctx, "voxel backprojection");
+}
+
+// Whether the sweep can run in the z-quad layout for subsets of up to
+// `subset` views: fast FP32 arithmetic, reduced or FP32 storage under 4 GiB
+// (the correction's 32-bit buffer offsets) and an LDS-staged update.
+bool sweep_zq_eligible(const ct_context *ctx, const ct_geometry &g, ct_dtype vdt, int subset) {
+    if (vdt == CT_F64 || ctx->math != CT_MATH_FAST) return false;
+    const DGeo<float> d = derive<float>(g);
+    const uint64_t bytes = (uint64_t)d.M * d.N * (uint64_t)((d.Z + 3) & ~3) * dtype_size(vdt);
+    if (bytes >= (1ull << 32)) return false;
+    for (int nv : {1, subset})
+        if (!lds_eligible(ctx, d, nv * d.NS)) return false;
+    return true;
+}
+
+// SART update of a z-quad-layout volume (sweep_zq_eligible holds)
+ct_status launch_voxel_bp_zq(ct_context *ctx, const ct_geometry &g, void *volq, ct_dtype vdt, const void *corr,
+                             int32_t first, int32_t stride, int32_t nv, double beta) {
+    if (nv <= 0) return CT_OK;
+    ct_status st;
+    const float *ang = (const float *)angle_table(ctx, g, false, false, &st);
+    if (!ang) return st;
+    const DGeo<float> d = derive<float>(g);
+    switch (vdt) {
+    case CT_F32: return run_bp_lds<float, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
+    case CT_F16: return run_bp_lds<__half, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
+    case CT_BF16: return run_bp_lds<__hip_bfloat16, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
+    default: break;
+    }
+    return fail(ctx, CT_ERR_DTYPE, "z-quad update: unsupported volume dtype");
 }
 
 ct_status launch_voxel_bp(ct_context *ctx, const ct_geometry &g, void *vol, ct_dtype vdt, const void *corr,

@@ -387,6 +387,28 @@ def test_reduced_precision_storage(ctx, dt):
     assert rel(got16, ref) <= (1e-3 if dt == torch.float16 else 8e-3)
 
 
+@pytest.mark.parametrize("dt,tol", [(torch.float16, 3e-3), (torch.bfloat16, 1.2e-2)])
+@pytest.mark.parametrize("name", ["small", "aniso", "multisource"])
+@pytest.mark.parametrize("mode,nv", [(0, 1), (1, 1), (1, 3)])
+def test_backproject_reduced_precision(ctx, dt, tol, name, mode, nv):
+    """FP16/BF16 volumes take the packed-half chord engine: chord intervals in
+    voxel-local FP16 units, products accumulated in FP32.  Against the FP32
+    oracle on the storage-rounded input, the error is that of the stored
+    result (FP16 5e-4, BF16 4e-3 relative) plus the engine's ~1e-3 on W/L."""
+    g = GEOMS[name]
+    rng = np.random.default_rng(8)
+    x = torch.as_tensor(phantom(g, seed=18, scale=0.3).numpy()).to(dt)
+    x_rounded = x.float().numpy()
+    corr = rng.standard_normal((nv, *g.proj_shape[1:])).astype(np.float32) * 0.01
+    ref = orc.backproject_voxel(g, x_rounded, corr, view0=2, nviews=nv, beta=0.7, mode=mode, prec=32)
+    v = x.cuda()
+    ctx.backproject_voxel(g, v, dev(corr), view0=2, nviews=nv, beta=0.7, mode=mode)
+    got = host(v)
+    r = rel(got, ref)
+    print(f"{name} {dt} mode {mode} nv {nv}: rel {r:.3e}")
+    assert r <= tol
+
+
 def test_compat_layer(ctx):
     lib = core.lib()
     g = GEOMS["small"]
