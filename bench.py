@@ -138,6 +138,8 @@ def main() -> int:
     ap.add_argument("--pmc-sample", type=int, default=0,
                     help="counter-collection mode: a SART sweep over the first N views at full shape plus a "
                          "1 GiB device copy (calibration); exits without the JSON line")
+    ap.add_argument("--pmc-dtype", choices=("f32", "f16"), default="f32",
+                    help="storage type of the counter-collection sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -159,7 +161,8 @@ def main() -> int:
         # profiled run has few dispatches besides the kernels of interest
         zz, yy, xx = torch.meshgrid(*(torch.arange(n, device="cuda", dtype=torch.float32) - n / 2 + 0.5
                                       for n in g.vol_shape), indexing="ij")
-        ball = ((xx * xx + yy * yy + zz * zz) < (0.4 * g.nx) ** 2).to(torch.float32) * 0.02
+        sdt = torch.float16 if args.pmc_dtype == "f16" else torch.float32
+        ball = (((xx * xx + yy * yy + zz * zz) < (0.4 * g.nx) ** 2).to(torch.float32) * 0.02).to(sdt)
         gs = g.with_views(args.pmc_sample)      # the first N views of the orbit
         meas = ctx.forward(gs, ball)
         # calibration: the library's own element copy (k_convert F32 -> F32) of
@@ -168,7 +171,7 @@ def main() -> int:
         dst = torch.empty_like(src)
         ctx.convert(src, dst)
         torch.cuda.synchronize()
-        vol = torch.zeros(g.vol_shape, device="cuda")
+        vol = torch.zeros(g.vol_shape, device="cuda", dtype=sdt)
         ctx.sart_sweep(gs, vol, meas, beta=beta, subset=1)      # the sweep's kernels, as timed
         torch.cuda.synchronize()
         log(f"pmc sample done: {args.pmc_sample} views")

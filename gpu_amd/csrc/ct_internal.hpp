@@ -40,6 +40,10 @@ template <typename S> __device__ __forceinline__ typename LdT<S>::type ldf(const
     return St<S>::ld(p, i);
 }
 
+// hardware reciprocal and square root of the fast kernels
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Hardware min/max for the fast kernels.  v_min_f32/v_max_f32/v_min3_f32
 // return the other operand for a quiet NaN input, which is the fminf/fmaxf
 // result for every NaN that arithmetic can produce (0*inf); issuing them
@@ -53,6 +57,12 @@ __device__ __forceinline__ float hmin(float a, float b) {
 __device__ __forceinline__ float hmax(float a, float b) {
     float r;
     asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// max(a - b, 0) for a - b < 1 (the VOP3 output clamp saturates to [0, 1])
+__device__ __forceinline__ float hsub_sat(float a, float b) {
+    float r;
+    asm("v_sub_f32_e64 %0, %1, %2 clamp" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
 __device__ __forceinline__ float hmin3(float a, float b, float c) {

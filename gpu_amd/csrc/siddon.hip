@@ -13,6 +13,8 @@
 // one branch-free step here: every axis whose next crossing equals the
 // minimum steps, which is the same set of cases without wave divergence.
 #include "ct_internal.hpp"
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 namespace ct {
@@ -313,6 +315,39 @@ __device__ __forceinline__ T siddon_sum(const DGeo<T> &d, T s, T c, int col, int
 // a constant; a z step moves it by +-1 inside a quad and by +-(4MN - 3)
 // across a quad boundary, i.e. when the plane crossed has an index that is a
 // multiple of 4.
+// One axis of the fast walk's plane model: plane n is crossed at
+// alpha = fma(n, k, b0) with k = v/(x2 - x1) and b0 = (b - x1)/(x2 - x1).  An
+// axis whose ray component is below 1e-6 of a voxel (the reference's parallel
+// test) never steps; the ray then stays in slab i0 = floor((x1 - b)/v).
+// The matched gather adjoint evaluates the same planes per detector column
+// and row, so every segment is the same number in both.
+struct PlaneAxis {
+    float k, b0;
+    int sg, i0;
+    bool par;
+    // clip [lo, hi] to the grid's planes 0 and n
+    __device__ __forceinline__ void clip(float &lo, float &hi, int n) const {
+        if (par) return;
+        const float g0 = b0, gn = fmaf((float)n, k, b0);
+        lo = hmax(lo, hmin(g0, gn));
+        hi = hmin(hi, hmax(g0, gn));
+    }
+    // the slab holding alpha lo, starting from the reference setup's index
+    __device__ __forceinline__ bool start(int v0, float lo, int n, int &v) const {
+        if (par) { v = i0; return (unsigned)v < (unsigned)n; }
+        v = v0;
+        if (fmaf((float)(v + (sg < 0)), k, b0) > lo) v -= sg;        // lo lies before the entry plane
+        else if (fmaf((float)(v + (sg > 0)), k, b0) < lo) v += sg;   // lo lies past the exit plane
+        return (unsigned)v < (unsigned)n;
+    }
+};
+__device__ __forceinline__ PlaneAxis plane_axis(float v, float iv, float b, float x1, float id, int sg, bool par) {
+    PlaneAxis a;
+    a.k = v * id; a.b0 = (b - x1) * id; a.sg = sg; a.par = par;
+    a.i0 = (int)floorf((x1 - b) * iv);
+    return a;
+}
+
 template <int ES, bool ZQ = false> struct FastWalk {
     float kx, ky, kz, bx0, by0, bz0, sgxf, sgyf, sgzf, nfx, nfy, nfz, ax, ay, az, ac, amax, dist, len;
     int dix, diy, diz, dizb;
@@ -320,38 +355,39 @@ template <int ES, bool ZQ = false> struct FastWalk {
     // returns false for an empty ray (the forward yields 0 for it)
     __device__ __forceinline__ bool init(const DGeo<float> &d, float s, float c, int col, int row, int src) {
         RayState<float> r = ray_setup(d, s, c, col, row, src);
+        if (!r.hit) return false;
         const unsigned M = d.M, N = d.N, Z = d.Z, MN = M * N;
-        if (!(r.hit && (unsigned)r.vi < M && (unsigned)r.vj < N && (unsigned)r.vk < Z)) return false;
-        kx = d.vx * r.ixd; ky = d.vy * r.iyd; kz = d.vz * r.izd;
-        bx0 = (d.bx - r.x1x) * r.ixd; by0 = (d.by - r.x1y) * r.iyd; bz0 = (d.bz - r.x1z) * r.izd;
+        const PlaneAxis X = plane_axis(d.vx, d.ivx, d.bx, r.x1x, r.ixd, r.sgx, d.vx * 1e-6f > fabsf(r.x1x - (d.sx * c - d.sy * s)));
+        const PlaneAxis Y = plane_axis(d.vy, d.ivy, d.by, r.x1y, r.iyd, r.sgy, d.vy * 1e-6f > fabsf(r.x1y - (d.sx * s + d.sy * c)));
+        const PlaneAxis Zx = plane_axis(d.vz, d.ivz, d.bz, r.x1z, r.izd, r.sgz,
+                                        d.vz * 1e-6f > fabsf(r.x1z - (d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0)));
+        // the walk runs over [alpha_min, alpha_max] of the reference setup,
+        // clipped to the grid's entry and exit planes as this walk computes
+        // them (so no step past the grid carries a nonzero length)
+        float lo = r.ac, hi = r.amax;
+        X.clip(lo, hi, (int)M); Y.clip(lo, hi, (int)N); Zx.clip(lo, hi, (int)Z);
+        if (!(hi > lo)) return false;
+        // start voxel: the slab that holds alpha_min under these planes
+        int vi, vj, vk;
+        if (!X.start(r.vi, lo, (int)M, vi) || !Y.start(r.vj, lo, (int)N, vj) || !Zx.start(r.vk, lo, (int)Z, vk))
+            return false;
+        // a parallel axis gets k = 0, b0 = +inf: its pending crossing is +inf
+        kx = X.par ? 0.0f : X.k; bx0 = X.par ? __builtin_inff() : X.b0;
+        ky = Y.par ? 0.0f : Y.k; by0 = Y.par ? __builtin_inff() : Y.b0;
+        kz = Zx.par ? 0.0f : Zx.k; bz0 = Zx.par ? __builtin_inff() : Zx.b0;
         sgxf = (float)r.sgx; sgyf = (float)r.sgy; sgzf = (float)r.sgz;
-        nfx = (float)r.nx; nfy = (float)r.ny; nfz = (float)r.nz;
+        nfx = (float)(vi + (r.sgx > 0)); nfy = (float)(vj + (r.sgy > 0)); nfz = (float)(vk + (r.sgz > 0));
+        ax = fmaf(nfx, kx, bx0); ay = fmaf(nfy, ky, by0); az = fmaf(nfz, kz, bz0);
         if constexpr (ZQ) {
             dix = r.sgx * 4 * ES; diy = r.sgy * (int)M * 4 * ES;
             diz = r.sgz * ES; dizb = r.sgz * (4 * (int)MN - 3) * ES;
-            idx = (((unsigned)(r.vk >> 2) * MN + (unsigned)r.vj * M + (unsigned)r.vi) * 4u + ((unsigned)r.vk & 3u)) *
-                  (unsigned)ES;
+            idx = (((unsigned)(vk >> 2) * MN + (unsigned)vj * M + (unsigned)vi) * 4u + ((unsigned)vk & 3u)) * (unsigned)ES;
         } else {
             dix = r.sgx * ES; diy = r.sgy * (int)M * ES; diz = r.sgz * (int)MN * ES; dizb = diz;
-            idx = ((unsigned)r.vk * MN + (unsigned)r.vj * M + (unsigned)r.vi) * (unsigned)ES;
+            idx = ((unsigned)vk * MN + (unsigned)vj * M + (unsigned)vi) * (unsigned)ES;
         }
-        ax = r.ax; ay = r.ay; az = r.az; ac = r.ac; dist = r.dist;
-        // end the walk at the grid's exit planes as this walk computes them,
-        // so no step past the grid carries a nonzero length
-        amax = r.amax;
-        if (r.ax != __builtin_inff()) {
-            const int n = r.sgx > 0 ? (int)M : 0;
-            amax = hmin(amax, n == r.nx ? r.ax : fmaf((float)n, kx, bx0));
-        }
-        if (r.ay != __builtin_inff()) {
-            const int n = r.sgy > 0 ? (int)N : 0;
-            amax = hmin(amax, n == r.ny ? r.ay : fmaf((float)n, ky, by0));
-        }
-        if (r.az != __builtin_inff()) {
-            const int n = r.sgz > 0 ? (int)Z : 0;
-            amax = hmin(amax, n == r.nz ? r.az : fmaf((float)n, kz, bz0));
-        }
-        len = dist * (amax - ac);
+        ac = lo; amax = hi; dist = r.dist;
+        len = dist * (hi - lo);
         return true;
     }
     // one step: the segment's alpha length (times dist = its length in mm)
@@ -365,11 +401,10 @@ template <int ES, bool ZQ = false> struct FastWalk {
         const bool sx = ax == m, sy = ay == m, sz = az == m;
         // ZQ: the z plane about to be crossed bounds a quad
         const bool qb = ZQ && __builtin_amdgcn_fractf(nfz * 0.25f) == 0.0f;
-        const float nx1 = nfx + sgxf, ny1 = nfy + sgyf, nz1 = nfz + sgzf;
-        const float ax1 = fmaf(nx1, kx, bx0), ay1 = fmaf(ny1, ky, by0), az1 = fmaf(nz1, kz, bz0);
-        nfx = sx ? nx1 : nfx; ax = sx ? ax1 : ax;
-        nfy = sy ? ny1 : nfy; ay = sy ? ay1 : ay;
-        nfz = sz ? nz1 : nfz; az = sz ? az1 : az;
+        // the stepped axes move to their next plane; every pending crossing
+        // is re-evaluated from its plane index (the same value when unmoved)
+        nfx += sx ? sgxf : 0.0f; nfy += sy ? sgyf : 0.0f; nfz += sz ? sgzf : 0.0f;
+        ax = fmaf(nfx, kx, bx0); ay = fmaf(nfy, ky, by0); az = fmaf(nfz, kz, bz0);
         idx += (unsigned)((sx ? dix : 0) + (sy ? diy : 0) + (sz ? (qb ? dizb : diz) : 0));
         ac = m;
         return dl;
@@ -632,6 +667,305 @@ __global__ __launch_bounds__(256) void k_adjoint(DGeo<T> d, const T *__restrict_
     }
 }
 
+// ------------------------------------------------- gather form of the adjoint
+// Aᵀy without atomics, for the fast walk.  The walk's segment in voxel
+// (i, j, k) is max(0, min(exit planes, alpha_hi) - max(entry planes,
+// alpha_lo)) - it is in that voxel exactly while it is inside all three
+// slabs - and every term is separable: the x/y plane models and the x/y parts
+// of [alpha_lo, alpha_hi] depend on the detector column, the z ones on the
+// row.  So each voxel gathers over the pixels of its detector footprint
+// exactly the numbers the walk scatters, in any order and without atomics.
+//
+// Per-view tables: column  (kx, bx0, ky, by0) (lo_xy, hi_xy, i0x|par, i0y|par)
+//                  row     (kz, bz0, lo_z, hi_z) (i0z|par)
+//                  pixel   w = dist*y (0 for a ray the forward zeroes)
+// A parallel axis is flagged by i0 >= 0 (the slab the ray stays in); -1
+// marks a stepping axis.
+struct AdjCol { float4 a; float4 b; };
+
+__device__ __forceinline__ void axis_bounds(float v, float b, int n, float x1, float x2, float &k, float &b0, float &lo,
+                                            float &hi, int &i0, float iv) {
+    // the reference setup's slab interval along this axis
+    // (V63/kernel_IterativeRecon_universal.cu:77-104)
+    const float id = 1.0f / (x2 - x1);
+    const float a0 = (b - x1) * id, a1 = (b + v * (float)n - x1) * id;
+    lo = fminf(a0, a1); hi = fmaxf(a0, a1);
+    const bool par = v * 1e-6f > fabsf(x1 - x2);
+    const PlaneAxis P = plane_axis(v, iv, b, x1, id, 0, par);
+    k = P.k; b0 = P.b0;
+    i0 = par ? P.i0 : -1;
+    if (!par) {
+        const float g0 = b0, gn = fmaf((float)n, k, b0);
+        lo = hmax(lo, hmin(g0, gn));
+        hi = hmin(hi, hmax(g0, gn));
+    }
+}
+
+// tables for images [0, nimg) of views view0 + q (q = img / NS)
+template <typename SP>
+__global__ __launch_bounds__(256) void k_adj_prep(DGeo<float> d, const float *__restrict__ ang, int view0,
+                                                  const SP *__restrict__ proj, int nimg, float *__restrict__ pixw,
+                                                  AdjCol *__restrict__ colT, float4 *__restrict__ rowA,
+                                                  int *__restrict__ rowB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, img = blockIdx.z;
+    if (i >= d.R) return;
+    const int q = img / d.NS, src = img - q * d.NS;
+    const int v = view0 + q;
+    const float s = ang[2 * v], c = ang[2 * v + 1];
+    const int64_t P = (int64_t)d.R * d.ZP;
+    const float y = (float)ldf(proj, (int64_t)img * P + (int64_t)j * d.R + i);
+    float w = 0.0f;
+    if (y != 0.0f) {
+        FastWalk<1> fw;
+        if (fw.init(d, s, c, i, j, src) && !(fw.len < d.vz * 1e-6f)) w = fw.dist * y;
+    }
+    pixw[(int64_t)img * P + (int64_t)j * d.R + i] = w;
+    // the detector pixel and source, as ray_setup forms them
+    const float x2x = d.sx * c - d.sy * s, x2y = d.sx * s + d.sy * c;
+    if (j == 0 && src == 0) {
+        const float u = d.ymin + (float)i * d.du;
+        const float x1x = d.dod * c - u * s, x1y = d.dod * s + u * c;
+        float kx, bx0, lx, hx, ky, by0, ly, hy;
+        int ix0, iy0;
+        axis_bounds(d.vx, d.bx, d.M, x1x, x2x, kx, bx0, lx, hx, ix0, d.ivx);
+        axis_bounds(d.vy, d.by, d.N, x1y, x2y, ky, by0, ly, hy, iy0, d.ivy);
+        AdjCol e;
+        e.a = make_float4(kx, bx0, ky, by0);
+        e.b = make_float4(hmax(lx, ly), hmin(hx, hy), __int_as_float(ix0), __int_as_float(iy0));
+        colT[(int64_t)q * d.R + i] = e;
+    }
+    if (i == 0) {
+        const float x1z = d.zmin + (float)j * d.dv;
+        const float x2z = d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0;
+        float kz, bz0, lz, hz;
+        int iz0;
+        axis_bounds(d.vz, d.bz, d.Z, x1z, x2z, kz, bz0, lz, hz, iz0, d.ivz);
+        rowA[(int64_t)img * d.ZP + j] = make_float4(kz, bz0, lz, hz);
+        rowB[(int64_t)img * d.ZP + j] = iz0;
+    }
+}
+
+// slab interval [in, out] of voxel index n along one axis for a ray
+__device__ __forceinline__ void slab(float k, float b0, int i0, int n, float &in, float &out) {
+    if (i0 >= 0) {   // parallel: inside slab i0 for all alpha, else never
+        const bool hit = n == i0;
+        in = hit ? -__builtin_inff() : __builtin_inff();
+        out = hit ? __builtin_inff() : -__builtin_inff();
+    } else {
+        const float a0 = fmaf((float)n, k, b0), a1 = fmaf((float)(n + 1), k, b0);
+        in = hmin(a0, a1);
+        out = hmax(a0, a1);
+    }
+}
+
+constexpr int AZR = 8;          // z-run per lane
+constexpr int ALW = 80;         // staged footprint columns
+constexpr int ALP = ALW + 8;    // row pitch (zero columns for the NC overrun)
+constexpr int ALH = 32;         // staged footprint rows
+constexpr int ANC = 8;          // bounding-box columns per voxel
+
+template <int NC>
+__device__ __forceinline__ float adj_rows(const float *__restrict__ sw, const float4 *__restrict__ sra,
+                                          const int *__restrict__ srb, int z0, int z1, int cbase, int iz,
+                                          const float *xin, const float *xout) {
+    float acc = 0.0f;
+    for (int j = z0; j <= z1; ++j) {
+        const float4 ra = sra[j];
+        const float *prow = sw + j * ALP + cbase;
+        float pw[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) pw[cc] = prow[cc];
+        float zi, zo;
+        slab(ra.x, ra.y, srb[j], iz, zi, zo);
+        zi = hmax(zi, ra.z);
+        zo = hmin(zo, ra.w);
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+            const float seg = hsub_sat(hmin(xout[cc], zo), hmax(xin[cc], zi));   // alpha lengths are below 1
+            acc = fmaf(seg, pw[cc], acc);
+        }
+    }
+    return acc;
+}
+
+// one lane per (x, y) and its AZR z-voxels; a 16x16 (x, y) tile per
+// workgroup; per image the tile's detector footprint of weights is staged in
+// LDS.  Each view's per-voxel sum (FP32, ~10 terms) is added into an FP64
+// total; the volume is written once.
+template <typename SO>
+__global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const float *__restrict__ ang, int view0,
+                                                        int nimg, const float *__restrict__ pixw,
+                                                        const AdjCol *__restrict__ colT,
+                                                        const float4 *__restrict__ rowA, const int *__restrict__ rowB,
+                                                        SO *__restrict__ vol) {
+    __shared__ float sw[ALH * ALP];
+    __shared__ AdjCol sc[ALP];
+    __shared__ float4 sra[ALH];
+    __shared__ int srb[ALH];
+    const int tid = threadIdx.y * 16 + threadIdx.x;
+    const int3 tile = xcd_tile();
+    const int ix = tile.x * 16 + threadIdx.x, iy = tile.y * 16 + threadIdx.y, iz0 = tile.z * AZR;
+    const int nz = min(AZR, d.Z - iz0);
+    const bool active = ix < d.M && iy < d.N;
+    float cx = 0, cy = 0;
+    if (!d.tb) {
+        cx = d.bx + d.vx * 0.5f + (float)ix * d.vx;
+        cy = d.by + d.vy * 0.5f + (float)iy * d.vy;
+    } else {
+        cx = d.bx + d.vx * (0.5f + (float)ix);
+        cy = d.by + d.vy * (0.5f + (float)iy);
+    }
+    const float hvx = 0.5f * d.vx, hvy = 0.5f * d.vy, hvz = 0.5f * d.vz;
+    const float xl = cx - hvx, xh = cx + hvx, yl = cy - hvy, yh = cy + hvy;
+    const int tx0 = tile.x * 16, ty0 = tile.y * 16;
+    const float TX0 = d.bx + (float)tx0 * d.vx, TX1 = d.bx + (float)min(tx0 + 16, d.M) * d.vx;
+    const float TY0 = d.by + (float)ty0 * d.vy, TY1 = d.by + (float)min(ty0 + 16, d.N) * d.vy;
+    const float TZ0 = d.bz + (float)iz0 * d.vz, TZ1 = d.bz + (float)(iz0 + nz) * d.vz;
+    double W[AZR];
+#pragma unroll
+    for (int k = 0; k < AZR; ++k) W[k] = 0.0;
+    const int64_t P = (int64_t)d.R * d.ZP;
+    for (int img = 0; img < nimg; ++img) {
+        const int q = img / d.NS, src = img - q * d.NS;
+        const int v = view0 + q;
+        const float s = ang[2 * v], c = ang[2 * v + 1];
+        const float szs = d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0;
+        // this lane's voxel column: corner ratios and detector-column box
+        float rmin = __builtin_inff(), rmax = -__builtin_inff(), minY = __builtin_inff(), maxY = -__builtin_inff();
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                const float px = ii ? xh : xl, py = jj ? yh : yl;
+                const float t = px * c + py * s;
+                const float sv = -px * s + py * c;
+                const float ratio = (t - d.dod) * frcp(d.dso - t);
+                const float u = fmaf(ratio, sv - d.sy, sv);
+                minY = fminf(minY, u); maxY = fmaxf(maxY, u);
+                rmin = fminf(rmin, ratio); rmax = fmaxf(rmax, ratio);
+            }
+        const int y0r = (int)floorf(fmaf(minY - d.ymin, d.idu, 0.5f));
+        const int y1r = (int)floorf(fmaf(maxY - d.ymin, d.idu, 0.5f));
+        // the tile's footprint
+        float Tumin = __builtin_inff(), Tumax = -__builtin_inff(), Trmin = __builtin_inff(), Trmax = -__builtin_inff();
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                const float px = ii ? TX1 : TX0, py = jj ? TY1 : TY0;
+                const float t = px * c + py * s;
+                const float sv = -px * s + py * c;
+                const float ratio = (t - d.dod) * frcp(d.dso - t);
+                const float u = fmaf(ratio, sv - d.sy, sv);
+                Tumin = fminf(Tumin, u); Tumax = fmaxf(Tumax, u);
+                Trmin = fminf(Trmin, ratio); Trmax = fmaxf(Trmax, ratio);
+            }
+        const int C0 = max((int)floorf(fmaf(Tumin - d.ymin, d.idu, 0.5f)) - 1, 0);
+        const int C1 = min(min((int)floorf(fmaf(Tumax - d.ymin, d.idu, 0.5f)) + 1, d.R - 1), C0 + ALW - 1);
+        const int ncw = C1 - C0 + 1;
+        const float e0 = TZ0 - szs, e1 = TZ1 - szs;
+        const float Tw0 = fminf(fmaf(Trmin, e0, TZ0), fmaf(Trmax, e0, TZ0));
+        const float Tw1 = fmaxf(fmaf(Trmin, e1, TZ1), fmaf(Trmax, e1, TZ1));
+        const int R0 = max((int)floorf(fmaf(Tw0 - d.zmin, d.idv, 0.5f)) - 1, 0);
+        const int R1 = min(min((int)floorf(fmaf(Tw1 - d.zmin, d.idv, 0.5f)) + 1, d.ZP - 1), R0 + ALH - 1);
+        const int nrow = R1 - R0 + 1;
+        if (ncw <= 0 || nrow <= 0) continue;   // uniform
+        __syncthreads();
+        {
+            const float *g = pixw + (int64_t)img * P;
+            for (int i = tid; i < nrow * ALP; i += 256) {
+                const int r = i / ALP, cl = i - r * ALP;
+                sw[i] = cl < ncw ? g[(int64_t)(R0 + r) * d.R + C0 + cl] : 0.0f;
+            }
+            if (tid < ALP) {
+                AdjCol e;
+                if (tid < ncw) e = colT[(int64_t)q * d.R + C0 + tid];
+                else { e.a = make_float4(0, 0, 0, 0); e.b = make_float4(0, 0, 0, 0); }
+                sc[tid] = e;
+            }
+            if (tid < nrow) {
+                sra[tid] = rowA[(int64_t)img * d.ZP + R0 + tid];
+                srb[tid] = rowB[(int64_t)img * d.ZP + R0 + tid];
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        if ((y0r < 0 && y1r < 0) || (y0r > d.R - 1 && y1r > d.R - 1)) continue;
+        const int y0 = max(y0r, C0), y1 = min(min(y1r, C1), y0 + ANC - 1);
+        const int ncol = y1 - y0 + 1;
+        if (ncol <= 0) continue;
+        const int cbase = y0 - C0;
+        int NC;
+        if (__builtin_amdgcn_ballot_w64(ncol > 6)) NC = 8;
+        else if (__builtin_amdgcn_ballot_w64(ncol > 4)) NC = 6;
+        else NC = 4;
+        // x/y slab interval of this voxel column for each footprint column
+        float xin[ANC], xout[ANC];
+#pragma unroll
+        for (int cc = 0; cc < ANC; ++cc) {
+            const AdjCol e = sc[min(cbase + cc, ALP - 1)];
+            float ai, ao, bi, bo;
+            slab(e.a.x, e.a.y, __float_as_int(e.b.z), ix, ai, ao);
+            slab(e.a.z, e.a.w, __float_as_int(e.b.w), iy, bi, bo);
+            const bool in = cc < ncol;
+            xin[cc] = in ? hmax(hmax(ai, bi), e.b.x) : __builtin_inff();
+            xout[cc] = in ? hmin(hmin(ao, bo), e.b.y) : -__builtin_inff();
+        }
+#pragma unroll
+        for (int k = 0; k < AZR; ++k) {
+            if (k >= nz) break;
+            const int iz = iz0 + k;
+            const float cz = !d.tb ? d.bz + d.vz * 0.5f + (float)iz * d.vz : d.bz + d.vz * (0.5f + (float)iz);
+            const float zl = cz - hvz, zh = cz + hvz;
+            const float el = zl - szs, eh = zh - szs;
+            const float wl = fmaf(el >= 0.0f ? rmin : rmax, el, zl);
+            const float wh = fmaf(eh >= 0.0f ? rmax : rmin, eh, zh);
+            int z0 = (int)floorf(fmaf(wl - d.zmin, d.idv, 0.5f));
+            int z1 = (int)floorf(fmaf(wh - d.zmin, d.idv, 0.5f));
+            if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
+            z0 = max(z0, R0) - R0;
+            z1 = min(z1, R1) - R0;
+            float a;
+            if (NC == 8) a = adj_rows<8>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
+            else if (NC == 6) a = adj_rows<6>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
+            else a = adj_rows<4>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
+            W[k] += (double)a;
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < AZR; ++k) {
+        if (k >= nz) break;
+        St<SO>::st(vol, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, (float)W[k]);
+    }
+}
+
+// Host bound for k_adjoint_gather: every tile footprint fits ALW x ALH and
+// every voxel box ANC columns (the bound of bp_lds_fits without the
+// reconstruction cylinder: the adjoint covers every voxel).
+static bool adj_gather_fits(const DGeo<float> &d) {
+    const double vx = d.vx, vy = d.vy, vz = d.vz, dso = d.dso, dod = d.dod;
+    const double xs[2] = {(double)d.bx, (double)d.bx + d.M * vx}, ys[2] = {(double)d.by, (double)d.by + d.N * vy};
+    double rho = 0;
+    for (double x : xs)
+        for (double y : ys) rho = std::max(rho, std::sqrt(x * x + y * y));
+    if (!(dso - rho > 0)) return false;
+    const double r_lo = (-rho - dod) / (dso + rho), r_hi = (rho - dod) / (dso - rho);
+    const double rmax = std::max(std::fabs(r_lo), std::fabs(r_hi));
+    const double dr = (dso - dod) / ((dso - rho) * (dso - rho));
+    double zs_lo = d.sz0, zs_hi = d.sz0 + (double)(d.NS - 1) * d.sdz;
+    if (zs_lo > zs_hi) std::swap(zs_lo, zs_hi);
+    const double z_lo = d.bz, z_hi = (double)d.bz + d.Z * vz;
+    const double zrel = std::max({std::fabs(z_lo - zs_lo), std::fabs(z_lo - zs_hi), std::fabs(z_hi - zs_lo),
+                                  std::fabs(z_hi - zs_hi)});
+    const double lat = rho + std::fabs((double)d.sy);
+    auto cols = [&](double D) { return ((1.0 + rmax) * D + dr * D * lat) / (double)d.du; };
+    auto rows = [&](double D, double H) { return ((1.0 + rmax) * H + dr * D * zrel) / (double)d.dv; };
+    const double DT = std::hypot(16 * vx, 16 * vy), DV = std::hypot(vx, vy);
+    return cols(DT) + 4.0 <= ALW && rows(DT, AZR * vz) + 4.0 <= ALH && cols(DV) + 2.0 <= ANC;
+}
+
 // ---------------------------------------------------------------- launchers
 template <typename T, typename SV, typename SO, typename SM, int CMD>
 static ct_status run_forward(ct_context *ctx, const ct_geometry &g, const void *vol, void *out, const void *meas,
@@ -784,6 +1118,47 @@ static ct_status run_adjoint(ct_context *ctx, const ct_geometry &g, const void *
     return launch_check(ctx, "adjoint");
 }
 
+template <typename SO>
+static void launch_gather(ct_context *ctx, const DGeo<float> &d, const float *ang, int view0, int nimg,
+                          const float *pixw, const AdjCol *colT, const float4 *rowA, const int *rowB, void *vol) {
+    dim3 grid((d.M + 15) / 16, (d.N + 15) / 16, (d.Z + AZR - 1) / AZR);
+    hipLaunchKernelGGL((k_adjoint_gather<SO>), grid, dim3(16, 16, 1), 0, ctx->stream, d, ang, view0, nimg, pixw, colT,
+                       rowA, rowB, (SO *)vol);
+}
+
+static ct_status run_adjoint_gather(ct_context *ctx, const ct_geometry &g, const DGeo<float> &d, const void *proj,
+                                    ct_dtype pdt, void *vol, ct_dtype vdt, int view0, int nv) {
+    ct_status st;
+    const float *ang = (const float *)angle_table(ctx, g, false, false, &st);
+    if (!ang) return st;
+    const int nimg = nv * d.NS;
+    const size_t P = (size_t)d.R * d.ZP;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pw_b = up(sizeof(float) * P * nimg), col_b = up(sizeof(AdjCol) * (size_t)d.R * nv);
+    const size_t ra_b = up(sizeof(float4) * (size_t)d.ZP * nimg), rb_b = up(sizeof(int) * (size_t)d.ZP * nimg);
+    char *tab = (char *)slot(ctx, SLOT_TMP_VOL, pw_b + col_b + ra_b + rb_b, &st);
+    if (!tab) return st;
+    float *pixw = (float *)tab;
+    AdjCol *colT = (AdjCol *)(tab + pw_b);
+    float4 *rowA = (float4 *)(tab + pw_b + col_b);
+    int *rowB = (int *)(tab + pw_b + col_b + ra_b);
+    ProfScope ps(ctx, CT_K_ADJOINT);
+    dim3 pg((d.R + 255) / 256, d.ZP, nimg);
+    switch (pdt) {
+    case CT_F32: hipLaunchKernelGGL((k_adj_prep<float>), pg, dim3(256), 0, ctx->stream, d, ang, view0, (const float *)proj, nimg, pixw, colT, rowA, rowB); break;
+    case CT_F16: hipLaunchKernelGGL((k_adj_prep<__half>), pg, dim3(256), 0, ctx->stream, d, ang, view0, (const __half *)proj, nimg, pixw, colT, rowA, rowB); break;
+    case CT_BF16: hipLaunchKernelGGL((k_adj_prep<__hip_bfloat16>), pg, dim3(256), 0, ctx->stream, d, ang, view0, (const __hip_bfloat16 *)proj, nimg, pixw, colT, rowA, rowB); break;
+    default: return fail(ctx, CT_ERR_DTYPE, "unsupported projection dtype");
+    }
+    switch (vdt) {
+    case CT_F32: launch_gather<float>(ctx, d, ang, view0, nimg, pixw, colT, rowA, rowB, vol); break;
+    case CT_F16: launch_gather<__half>(ctx, d, ang, view0, nimg, pixw, colT, rowA, rowB, vol); break;
+    case CT_BF16: launch_gather<__hip_bfloat16>(ctx, d, ang, view0, nimg, pixw, colT, rowA, rowB, vol); break;
+    default: return fail(ctx, CT_ERR_DTYPE, "unsupported volume dtype");
+    }
+    return launch_check(ctx, "adjoint");
+}
+
 ct_status launch_adjoint(ct_context *ctx, const ct_geometry &g, const void *proj, ct_dtype pdt, void *vol,
                          ct_dtype vdt, int32_t view0, int32_t nv) {
     const int64_t V = (int64_t)g.nx * g.ny * g.nz;
@@ -799,6 +1174,12 @@ ct_status launch_adjoint(ct_context *ctx, const ct_geometry &g, const void *proj
     // rounded once to the output type.  Fast mode accumulates in the z-quad
     // layout and unpacks while converting.
     const bool zq = ctx->math == CT_MATH_FAST;
+    if (zq) {
+        // gather form: the walk's segments summed per voxel, no atomics
+        const DGeo<float> d = derive<float>(g);
+        const int nimg = nv * d.NS;
+        if (nimg <= 65535 && adj_gather_fits(d)) return run_adjoint_gather(ctx, g, d, proj, pdt, vol, vdt, view0, nv);
+    }
     const int64_t Va = zq ? (int64_t)g.nx * g.ny * ((g.nz + 3) & ~3) : V;
     ct_status st;
     double *acc = (double *)slot(ctx, SLOT_TMP_VOL, sizeof(double) * Va, &st);

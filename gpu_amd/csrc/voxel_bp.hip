@@ -157,8 +157,6 @@ __device__ __forceinline__ void voxel_wl(const DGeo<T> &d, T cx, T cy, T cz, T s
 // that lies exactly in a voxel boundary plane (zero divisor) then resolves
 // exactly as in the reference (x/0 = inf, 0/0 = NaN), and the remaining
 // ~1 ulp differences only move chords shorter than the 1e-6*vx cut.
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, float cy, float cz, float s, float c,
                                               float szs, const float *__restrict__ corr, float &W, float &L) {
@@ -341,10 +339,11 @@ __device__ __forceinline__ unsigned pk_max_bc(unsigned a) {
     asm("v_pk_max_f16 %0, %1, %1 op_sel:[0,1] op_sel_hi:[1,0]" : "=v"(r) : "v"(a));
     return r;
 }
-// max(a - b, 0) per half
+// max(a - b, 0) per half, for a - b < 1 (output clamp to [0, 1]); local
+// spans are at most (DSO - DOD)/|S - P| <= 1
 __device__ __forceinline__ unsigned pk_span(unsigned a, unsigned b) {
     unsigned r;
-    asm("v_pk_add_f16 %0, %1, %2 neg_lo:[0,1] neg_hi:[0,1]\n\tv_pk_max_f16 %0, %0, 0" : "=&v"(r) : "v"(a), "v"(b));
+    asm("v_pk_add_f16 %0, %1, %2 neg_lo:[0,1] neg_hi:[0,1] clamp" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
 __device__ __forceinline__ float dot2(unsigned a, unsigned b, float c) {
@@ -405,7 +404,8 @@ __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const flo
         const float loz = hmin(az0, az1), hiz = hmax(az0, az1);
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
-            const float span = hmax(hmin(hi[cc], hiz) - hmax(lo[cc], loz), 0.0f);
+            // alpha spans are below 1 (a chord is a fraction of |S - P|)
+            const float span = hsub_sat(hmin(hi[cc], hiz), hmax(lo[cc], loz));
             wk = fmaf(span, pv[cc].x, wk);
             lk = fmaf(span, pv[cc].y, lk);
         }

@@ -227,6 +227,20 @@ def test_adjoint_dot_product_adjoint_64(ctx, dtype, tol):
     assert err <= tol
 
 
+@pytest.mark.parametrize("name", ["small", "offsets", "multisource", "fine"])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-6)])
+def test_adjoint_dot_product_paths(ctx, name, dtype, tol):
+    """Fast-mode FP32 adjoints: the gather form (every voxel sums the walk's
+    own segments over its detector footprint, no atomics) on the small
+    geometries, the atomic scatter of the same walk on "fine" (footprints too
+    wide for the gather's LDS window); FP64 takes the reference traversal."""
+    g = GEOMS[name]
+    lhs, rhs = ctx.adjoint_test(g, dtype=dtype, seed=77)
+    err = abs(lhs - rhs) / abs(lhs)
+    print(f"{name} {dtype}: rel {err:.3e}")
+    assert err <= tol
+
+
 def test_power_iteration_vs_dense_svd(ctx):
     g = Geometry(nx=6, ny=6, nz=4, vx=1.0, ncols=9, nrows=6, du=1.2, nviews=5, dso=40.0, dod=-20.0)
     n = g.nvox
