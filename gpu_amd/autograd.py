@@ -29,14 +29,16 @@ def default_context(device: int = 0) -> Context:
 
 class _Project(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, vol: torch.Tensor, g: Geometry, ct: Context, view0: int, nviews: int):
-        ctx.g, ctx.ct, ctx.view0, ctx.nviews = g, ct, view0, nviews
-        return ct.forward(g, vol.contiguous(), view0=view0, nviews=nviews)
+    def forward(ctx, vol: torch.Tensor, g: Geometry, ct: Context, view0: int, nviews: int,
+                out_dtype: torch.dtype | None):
+        ctx.g, ctx.ct, ctx.view0, ctx.nviews, ctx.vdtype = g, ct, view0, nviews, vol.dtype
+        return ct.forward(g, vol.contiguous(), view0=view0, nviews=nviews, out_dtype=out_dtype)
 
     @staticmethod
     def backward(ctx, grad: torch.Tensor):
-        gv = ctx.ct.adjoint(ctx.g, grad.contiguous(), view0=ctx.view0, nviews=ctx.nviews)
-        return gv, None, None, None, None
+        gv = torch.empty(ctx.g.vol_shape, device=grad.device, dtype=ctx.vdtype)
+        ctx.ct.adjoint(ctx.g, grad.contiguous(), vol=gv, view0=ctx.view0, nviews=ctx.nviews)
+        return gv, None, None, None, None, None
 
 
 class _TVEps(torch.autograd.Function):
@@ -54,11 +56,13 @@ class _TVEps(torch.autograd.Function):
 
 
 def project(vol: torch.Tensor, g: Geometry, ct: Context | None = None, view0: int = 0,
-            nviews: int | None = None) -> torch.Tensor:
-    """A x for views [view0, view0+nviews); differentiable (backward = A^T)."""
+            nviews: int | None = None, out_dtype: torch.dtype | None = None) -> torch.Tensor:
+    """A x for views [view0, view0+nviews); differentiable (backward = A^T).
+    The projections are stored in `out_dtype` (default: the volume's type);
+    ray sums accumulate in FP32/FP64 either way."""
     ct = ct or default_context(vol.device.index or 0)
     nv = g.nviews - view0 if nviews is None else nviews
-    return _Project.apply(vol, g, ct, view0, nv)
+    return _Project.apply(vol, g, ct, view0, nv, out_dtype)
 
 
 def tv_eps(vol: torch.Tensor, eps: float = 1e-8, ct: Context | None = None) -> torch.Tensor:

@@ -410,6 +410,32 @@ template <int ES, bool ZQ = false> struct FastWalk {
         return dl;
     }
     __device__ __forceinline__ bool ended() const { return !(amax - ac > loop_eps<float>()); }
+    // continue the walk from alpha a in [ac, amax]: the slab of each
+    // stepping axis that holds a (as start() finds it), its pending crossing
+    // and the voxel offset
+    __device__ __forceinline__ void restart(const DGeo<float> &d, float a) {
+        const int vi = restart_axis(kx, bx0, sgxf, nfx, a, d.M);
+        const int vj = restart_axis(ky, by0, sgyf, nfy, a, d.N);
+        const int vk = restart_axis(kz, bz0, sgzf, nfz, a, d.Z);
+        ax = fmaf(nfx, kx, bx0); ay = fmaf(nfy, ky, by0); az = fmaf(nfz, kz, bz0);
+        const unsigned M = d.M, MN = (unsigned)d.M * (unsigned)d.N;
+        if constexpr (ZQ)
+            idx = (((unsigned)(vk >> 2) * MN + (unsigned)vj * M + (unsigned)vi) * 4u + ((unsigned)vk & 3u)) * (unsigned)ES;
+        else
+            idx = ((unsigned)vk * MN + (unsigned)vj * M + (unsigned)vi) * (unsigned)ES;
+        ac = a;
+    }
+    __device__ __forceinline__ static int restart_axis(float k, float b0, float sg, float &nf, float a, int n) {
+        const int up = sg > 0.0f;
+        if (k == 0.0f) return (int)nf - up;   // parallel axis: its slab never changes
+        const int sgi = up ? 1 : -1;
+        int v = min(max((int)floorf((a - b0) / k), 0), n - 1);
+        if (fmaf((float)(v + 1 - up), k, b0) > a) v -= sgi;     // a lies before the entry plane
+        else if (fmaf((float)(v + up), k, b0) < a) v += sgi;    // a lies past the exit plane
+        v = min(max(v, 0), n - 1);
+        nf = (float)(v + up);
+        return v;
+    }
 };
 
 constexpr int FAST_B = 16;  // steps per batch: their gathers are in flight together
@@ -418,18 +444,19 @@ constexpr int FAST_B = 16;  // steps per batch: their gathers are in flight toge
 // each batch is summed in FP32 and the batch sums accumulate in FP64.
 // Volumes under 4 GiB are read with range-checked buffer loads on 32-bit
 // byte offsets; larger ones use 64-bit addresses with an explicit range test.
-template <typename SV, bool BUF, bool ZQ = false>
-__device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s, float c, int col, int row, int src,
-                                                  const SV *__restrict__ f, float &len_out) {
-    constexpr int ES = BUF ? (int)sizeof(SV) : 1;
-    FastWalk<ES, ZQ> w;
-    len_out = 0.0f;
-    if (!w.init(d, s, c, col, row, src)) return 0.0f;
+// The batch loop of the fast ray sum over an initialised walker: batches of
+// FAST_B steps whose loads are issued together, each batch summed in FP32 and
+// the batch sums accumulated in FP64.  Returns the sum of alpha-length x value
+// (times dist = the ray sum).  Volumes under 4 GiB are read with
+// range-checked buffer loads on 32-bit byte offsets; larger ones use 64-bit
+// addresses with an explicit range test.
+template <typename SV, bool BUF, bool ZQ>
+__device__ __forceinline__ double walk_sum(FastWalk<BUF ? (int)sizeof(SV) : 1, ZQ> &w, const DGeo<float> &d,
+                                           const SV *__restrict__ f) {
     // elements of the array (the z-quad layout pads z to a multiple of 4)
     const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (ZQ ? (unsigned)((d.Z + 3) & ~3) : (unsigned)d.Z);
     // (for the 64-bit path the resource is built but never used)
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(f, BUF ? nvox * (unsigned)sizeof(SV) : 0u);
-    // sum of alpha-length * value; times dist at the end
     double sum = 0.0;
     int guard = d.M + d.N + d.Z + 4;
     do {
@@ -449,8 +476,17 @@ __device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s,
         for (int u = 1; u < FAST_B; ++u) ps = fmaf(dl[u], fv[u], ps);
         sum += (double)ps;
     } while (!w.ended() && guard > 0);
+    return sum;
+}
+
+template <typename SV, bool BUF, bool ZQ = false>
+__device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s, float c, int col, int row, int src,
+                                                  const SV *__restrict__ f, float &len_out) {
+    FastWalk<BUF ? (int)sizeof(SV) : 1, ZQ> w;
+    len_out = 0.0f;
+    if (!w.init(d, s, c, col, row, src)) return 0.0f;
     len_out = w.len;
-    return (float)(sum * (double)w.dist);
+    return (float)(walk_sum<SV, BUF, ZQ>(w, d, f) * (double)w.dist);
 }
 
 template <typename SV>
@@ -511,21 +547,43 @@ __global__ __launch_bounds__(256) void k_correction_zq(DGeo<float> d, const floa
                                                        const SV *__restrict__ fq, float *__restrict__ out,
                                                        const SM *__restrict__ meas, int first, int stride, int mv0,
                                                        int z0) {
-    int col, row;
-    const int slot = tile_coords(col, row);
-    if (col >= d.R || row >= d.ZP) return;
-    const int img = slot + z0;
+    // a 16x8-pixel tile per workgroup: waves 0-1 walk each ray from alpha_min
+    // to the midpoint of its span, waves 2-3 from the midpoint on; the two
+    // partial sums meet in LDS.  Twice the waves of one lane per ray, and
+    // halves of equal length, keep more gathers in flight.
+    __shared__ double other[128];
+    const int tid = threadIdx.x, h = tid >> 7, rr = tid & 127;
+    const int w = rr >> 6, l = rr & 63;
+    const int3 t = xcd_tile();
+    const int col = t.x * 16 + w * 8 + (l & 7), row = t.y * 8 + (l >> 3);
+    const int img = t.z + z0;
+    const bool inb = col < d.R && row < d.ZP;
     const int q = img / d.NS, src = img - q * d.NS;
     const int v = first + q * stride;
-    const float s = ang[2 * v], c = ang[2 * v + 1];
-    float len;
-    const float sum = siddon_walk_fast<SV, true, true>(d, s, c, col, row, src, fq, len);
+    double sum = 0.0;
+    float len = 0.0f, dist = 0.0f;
+    if (inb) {
+        const float s = ang[2 * v], c = ang[2 * v + 1];
+        FastWalk<(int)sizeof(SV), true> fw;
+        if (fw.init(d, s, c, col, row, src)) {
+            len = fw.len;
+            dist = fw.dist;
+            const float amid = 0.5f * (fw.ac + fw.amax);
+            if (h == 0) fw.amax = amid;
+            else fw.restart(d, amid);
+            sum = walk_sum<SV, true, true>(fw, d, fq);
+        }
+    }
+    if (h == 1) other[rr] = sum;
+    __syncthreads();
+    if (h == 1 || !inb) return;
     const int64_t P = (int64_t)d.R * d.ZP;
     const int64_t pix = (int64_t)row * d.R + col;
     float r = 0.0f;
     if (!(len < d.vz * 1e-6f)) {
+        const float ax = (float)((sum + other[rr]) * (double)dist);
         const float b = (float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix);
-        r = (b - sum) / len;
+        r = (b - ax) / len;
     }
     out[(int64_t)img * P + pix] = r;
 }
@@ -1054,7 +1112,7 @@ static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const 
     ProfScope ps(ctx, CT_K_FORWARD);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
-        dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
+        dim3 grid((d.R + 15) / 16, (d.ZP + 7) / 8, nz);
         hipLaunchKernelGGL((k_correction_zq<SV, SM>), grid, dim3(256), 0, ctx->stream, d, ang, (const SV *)volq, corr,
                            (const SM *)meas, first, stride, 0, z0);
     }
