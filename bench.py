@@ -130,7 +130,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--cpu-views", type=int, default=8)
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 reference iterations (accuracy)")
     ap.add_argument("--accuracy", type=int, default=0,
                     help="accuracy mode: N SART-FISTA-TV iterations in FP64 (the GPU FP64 instantiation as the "
                          "oracle), FP32, FP16 and BF16; prints per-iteration RMSE / relative L2 against FP64 and "
@@ -285,6 +286,31 @@ def main() -> int:
                           "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof16.items() if v[1]}}
         log(f"FP16: {ms16 / 1e3:.3f} s/iter, GRays/s {grays16:.3f}, speedup {ms32 / ms16:.3f}, rmse {rmse:.3e}")
         del meas16
+
+    if not args.no_f64:
+        # error against the FP64 oracle: the FP64 instantiation of the same
+        # kernels, run for the same number of iterations on the same data
+        # (outside the timed region)
+        log(f"FP64 reference: {args.warmup + args.steps} iterations")
+        x64 = torch.zeros(g.vol_shape, device="cuda", dtype=torch.float64)
+        xp64 = torch.zeros_like(x64)
+        t64 = 1.0
+        for _ in range(args.warmup + args.steps):
+            t64 = ctx.sart_fista_iteration(g, x64, xp64, meas32, t64, beta=beta, subset=1, fgp_lambda=lam,
+                                           fgp_niter=niter)
+        torch.cuda.synchronize()
+        ref_n = float(torch.linalg.vector_norm(xp64))
+        acc = {"iterations": args.warmup + args.steps, "reference": "FP64 GPU instantiation of the same kernels"}
+        for k, xk in (("f32", x32), ("f16", x16 if not args.no_fp16 else None)):
+            if xk is None:
+                continue
+            d = xk.double() - xp64
+            acc[k] = {"rmse_per_mm": float(torch.sqrt((d * d).mean())),
+                      "rel_l2": float(torch.linalg.vector_norm(d)) / ref_n}
+        result["accuracy_vs_f64"] = acc
+        log(f"vs FP64: " + ", ".join(f"{k} rmse {v['rmse_per_mm']:.3e} rel {v['rel_l2']:.3e}"
+                                     for k, v in acc.items() if isinstance(v, dict)))
+        del x64, xp64
 
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_views} views")

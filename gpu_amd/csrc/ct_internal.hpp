@@ -44,6 +44,14 @@ template <typename S> __device__ __forceinline__ typename LdT<S>::type ldf(const
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Detector pixels whose centres lie in a voxel's shadow [a, b] (in pixels
+// from the centre of pixel 0): ceil(a - 0.01) .. floor(b + 0.01).  The chord
+// of a ray vanishes at the shadow's edge, so this keeps every pixel with a
+// nonzero chord and drops the empty margin of the reference's
+// floor(x + 0.5) bounding box (fast kernels only).
+__device__ __forceinline__ int shadow_lo(float a) { return (int)ceilf(a - 0.01f); }
+__device__ __forceinline__ int shadow_hi(float b) { return (int)floorf(b + 0.01f); }
+
 // Hardware min/max for the fast kernels.  v_min_f32/v_max_f32/v_min3_f32
 // return the other operand for a quiet NaN input, which is the fminf/fmaxf
 // result for every NaN that arithmetic can produce (0*inf); issuing them

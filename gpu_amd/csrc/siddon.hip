@@ -542,20 +542,24 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
 // SART correction (CMD 1) of the sweep on a z-quad-layout volume (fast
 // mode, FP32 arithmetic, volume under 4 GiB): the same rays, segments and
 // output as k_forward<float, SV, float, SM, 1, true>.
+// CORR_SPLIT parts per ray: a workgroup holds 256 / CORR_SPLIT rays (a
+// 16x8-pixel tile for 2 parts, 8x8 for 4); part p walks alpha in
+// [lo + (hi - lo) p / S, lo + (hi - lo) (p + 1) / S] and the partial sums meet
+// in LDS, added in part order.  More waves per view and parts of equal length
+// keep more gathers in flight.
+constexpr int CORR_SPLIT = 2;
+
 template <typename SV, typename SM>
 __global__ __launch_bounds__(256) void k_correction_zq(DGeo<float> d, const float *__restrict__ ang,
                                                        const SV *__restrict__ fq, float *__restrict__ out,
                                                        const SM *__restrict__ meas, int first, int stride, int mv0,
                                                        int z0) {
-    // a 16x8-pixel tile per workgroup: waves 0-1 walk each ray from alpha_min
-    // to the midpoint of its span, waves 2-3 from the midpoint on; the two
-    // partial sums meet in LDS.  Twice the waves of one lane per ray, and
-    // halves of equal length, keep more gathers in flight.
-    __shared__ double other[128];
-    const int tid = threadIdx.x, h = tid >> 7, rr = tid & 127;
+    constexpr int S = CORR_SPLIT, NR = 256 / S;
+    __shared__ double part[256];
+    const int tid = threadIdx.x, p = tid / NR, rr = tid % NR;
     const int w = rr >> 6, l = rr & 63;
     const int3 t = xcd_tile();
-    const int col = t.x * 16 + w * 8 + (l & 7), row = t.y * 8 + (l >> 3);
+    const int col = t.x * (S == 2 ? 16 : 8) + w * 8 + (l & 7), row = t.y * 8 + (l >> 3);
     const int img = t.z + z0;
     const bool inb = col < d.R && row < d.ZP;
     const int q = img / d.NS, src = img - q * d.NS;
@@ -568,20 +572,22 @@ __global__ __launch_bounds__(256) void k_correction_zq(DGeo<float> d, const floa
         if (fw.init(d, s, c, col, row, src)) {
             len = fw.len;
             dist = fw.dist;
-            const float amid = 0.5f * (fw.ac + fw.amax);
-            if (h == 0) fw.amax = amid;
-            else fw.restart(d, amid);
+            const float lo = fw.ac, span = fw.amax - fw.ac;
+            if (p + 1 < S) fw.amax = fmaf(span, (float)(p + 1) / (float)S, lo);
+            if (p > 0) fw.restart(d, fmaf(span, (float)p / (float)S, lo));
             sum = walk_sum<SV, true, true>(fw, d, fq);
         }
     }
-    if (h == 1) other[rr] = sum;
+    part[tid] = sum;
     __syncthreads();
-    if (h == 1 || !inb) return;
+    if (p != 0 || !inb) return;
+#pragma unroll
+    for (int k = 1; k < S; ++k) sum += part[k * NR + rr];
     const int64_t P = (int64_t)d.R * d.ZP;
     const int64_t pix = (int64_t)row * d.R + col;
     float r = 0.0f;
     if (!(len < d.vz * 1e-6f)) {
-        const float ax = (float)((sum + other[rr]) * (double)dist);
+        const float ax = (float)(sum * (double)dist);
         const float b = (float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix);
         r = (b - ax) / len;
     }
@@ -903,8 +909,8 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const flo
                 minY = fminf(minY, u); maxY = fmaxf(maxY, u);
                 rmin = fminf(rmin, ratio); rmax = fmaxf(rmax, ratio);
             }
-        const int y0r = (int)floorf(fmaf(minY - d.ymin, d.idu, 0.5f));
-        const int y1r = (int)floorf(fmaf(maxY - d.ymin, d.idu, 0.5f));
+        const int y0r = shadow_lo((minY - d.ymin) * d.idu);
+        const int y1r = shadow_hi((maxY - d.ymin) * d.idu);
         // the tile's footprint
         float Tumin = __builtin_inff(), Tumax = -__builtin_inff(), Trmin = __builtin_inff(), Trmax = -__builtin_inff();
 #pragma unroll
@@ -957,7 +963,8 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const flo
         int NC;
         if (__builtin_amdgcn_ballot_w64(ncol > 6)) NC = 8;
         else if (__builtin_amdgcn_ballot_w64(ncol > 4)) NC = 6;
-        else NC = 4;
+        else if (__builtin_amdgcn_ballot_w64(ncol > 3)) NC = 4;
+        else NC = 3;
         // x/y slab interval of this voxel column for each footprint column
         float xin[ANC], xout[ANC];
 #pragma unroll
@@ -979,15 +986,16 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const flo
             const float el = zl - szs, eh = zh - szs;
             const float wl = fmaf(el >= 0.0f ? rmin : rmax, el, zl);
             const float wh = fmaf(eh >= 0.0f ? rmax : rmin, eh, zh);
-            int z0 = (int)floorf(fmaf(wl - d.zmin, d.idv, 0.5f));
-            int z1 = (int)floorf(fmaf(wh - d.zmin, d.idv, 0.5f));
+            int z0 = shadow_lo((wl - d.zmin) * d.idv);
+            int z1 = shadow_hi((wh - d.zmin) * d.idv);
             if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
             z0 = max(z0, R0) - R0;
             z1 = min(z1, R1) - R0;
             float a;
             if (NC == 8) a = adj_rows<8>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
             else if (NC == 6) a = adj_rows<6>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
-            else a = adj_rows<4>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
+            else if (NC == 4) a = adj_rows<4>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
+            else a = adj_rows<3>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
             W[k] += (double)a;
         }
     }
@@ -1112,7 +1120,8 @@ static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const 
     ProfScope ps(ctx, CT_K_FORWARD);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
-        dim3 grid((d.R + 15) / 16, (d.ZP + 7) / 8, nz);
+        const int tw = CORR_SPLIT == 2 ? 16 : 8;
+        dim3 grid((d.R + tw - 1) / tw, (d.ZP + 7) / 8, nz);
         hipLaunchKernelGGL((k_correction_zq<SV, SM>), grid, dim3(256), 0, ctx->stream, d, ang, (const SV *)volq, corr,
                            (const SM *)meas, first, stride, 0, z0);
     }

@@ -501,8 +501,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                 minY = fminf(minY, u); maxY = fmaxf(maxY, u);
                 rmin = fminf(rmin, ratio); rmax = fmaxf(rmax, ratio);
             }
-        const int y0r = (int)floorf(fmaf(minY - d.ymin, d.idu, 0.5f));
-        const int y1r = (int)floorf(fmaf(maxY - d.ymin, d.idu, 0.5f));
+        const int y0r = shadow_lo((minY - d.ymin) * d.idu);
+        const int y1r = shadow_hi((maxY - d.ymin) * d.idu);
         const bool yoff = (y0r < 0 && y1r < 0) || (y0r > d.R - 1 && y1r > d.R - 1);
         // the tile's footprint: columns from its four (x, y) corners, rows
         // from its z faces at the extreme ratios
@@ -576,7 +576,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             if (__builtin_amdgcn_ballot_w64(ncol > 6)) NC = 8;
             else if (__builtin_amdgcn_ballot_w64(ncol > 5)) NC = 6;
             else if (__builtin_amdgcn_ballot_w64(ncol > 4)) NC = 5;
-            else NC = 4;
+            else if (__builtin_amdgcn_ballot_w64(ncol > 3)) NC = 4;
+            else NC = 3;
             float lo[NCMAX], hi[NCMAX];
 #pragma unroll
             for (int cc = 0; cc < NCMAX; ++cc) {
@@ -614,8 +615,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                 const float el = zl - szs, eh = zh - szs;
                 const float wl = fmaf(el >= 0.0f ? rmin : rmax, el, zl);
                 const float wh = fmaf(eh >= 0.0f ? rmax : rmin, eh, zh);
-                int z0 = (int)floorf(fmaf(wl - d.zmin, d.idv, 0.5f));
-                int z1 = (int)floorf(fmaf(wh - d.zmin, d.idv, 0.5f));
+                int z0 = shadow_lo((wl - d.zmin) * d.idv);
+                int z1 = shadow_hi((wh - d.zmin) * d.idv);
                 if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
                 z0 = max(z0, R0) - R0;
                 z1 = min(z1, R1) - R0;
@@ -628,7 +629,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                     else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                     else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                 }
                 if constexpr (ONE)
                     bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,

@@ -101,14 +101,17 @@ def _adam_cpu(g, b, lam, eps, lr, iters):
     return x, losses
 
 
-@pytest.mark.parametrize("storage,tol", [(torch.float32, 2e-3), (torch.bfloat16, 3e-2)])
-def test_multisource_adam_vs_fp64_cpu(ctx, storage, tol):
+@pytest.mark.parametrize("storage,loss_tol,tol", [(torch.float32, 1e-4, 2e-2), (torch.bfloat16, 2e-2, 6e-2)])
+def test_multisource_adam_vs_fp64_cpu(ctx, storage, loss_tol, tol):
     """Gradient reconstruction in the multi-source geometry at reduced scale
     (24^3 at 0.75 mm, 6 sources 3 mm apart, 12 views of 64x16 at 0.5 mm):
     Adam (lr 1e-3) on 0.5||Ax - b||^2 + 1e-3 TV_eps through the autograd
     projector (forward = ray sums, backward = matched adjoint), volume stored
     in `storage` with FP32 master weights, against the same recurrence in FP64
-    on the CPU oracle."""
+    on the CPU oracle.  The loss trajectories agree to `loss_tol` at every
+    iteration.  The volumes agree to `tol`, a looser bar: Adam's normalised
+    step turns rounding-level gradients of voxels that no ray constrains
+    into lr-sized steps, which dominate the volume difference (FP32 ~1e-2)."""
     from gpu_amd.geometry import Geometry
     g = Geometry(nx=24, ny=24, nz=24, vx=0.75, ncols=64, nrows=16, du=0.5, nviews=12,
                  nsources=6, src_z0=-7.5, src_dz=3.0)
@@ -133,5 +136,8 @@ def test_multisource_adam_vs_fp64_cpu(ctx, storage, tol):
     err = np.linalg.norm(got - x_ref) / np.linalg.norm(x_ref)
     print(f"{storage}: rel L2 vs FP64 CPU Adam {err:.3e}; loss {losses[0]:.4e} -> {losses[-1]:.4e} "
           f"(CPU {loss_ref[0]:.4e} -> {loss_ref[-1]:.4e})")
+    lerr = max(abs(a - b) / abs(b) for a, b in zip(losses, loss_ref))
+    print(f"max relative loss difference {lerr:.3e}")
     assert losses[-1] < 0.5 * losses[0]
+    assert lerr <= loss_tol
     assert err <= tol
