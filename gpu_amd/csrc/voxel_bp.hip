@@ -392,7 +392,7 @@ __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const fl
 
 template <int NC>
 __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
-                                        int cbase, float zl, float zh, const float *lo, const float *hi, float &wk,
+                                        int cbase, float zl, float vz, const float *lo, const float *hi, float &wk,
                                         float &lk) {
     for (int j = z0; j <= z1; ++j) {
         const float2 rt = sr[j];
@@ -400,7 +400,9 @@ __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const flo
         float2 pv[NC];
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) pv[cc] = prow[cc];
-        const float az0 = (zl - rt.x) * rt.y, az1 = (zh - rt.x) * rt.y;
+        // the two z-plane crossings of the row's ray; the upper one one voxel
+        // height further along
+        const float az0 = (zl - rt.x) * rt.y, az1 = fmaf(vz, rt.y, az0);
         const float loz = hmin(az0, az1), hiz = hmax(az0, az1);
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             float lo[NCMAX], hi[NCMAX];
 #pragma unroll
             for (int cc = 0; cc < NCMAX; ++cc) {
-                if (H && cc >= NC) { lo[cc] = __builtin_inff(); hi[cc] = -__builtin_inff(); continue; }
+                if (cc >= NC) { lo[cc] = __builtin_inff(); hi[cc] = -__builtin_inff(); continue; }
                 const float4 ct = sc[min(cbase + cc, LP - 1)];
                 const float ax0 = (xl - ct.x) * ct.z, ax1 = (xh - ct.x) * ct.z;
                 const float ay0 = (yl - ct.y) * ct.w, ay1 = (yh - ct.y) * ct.w;
@@ -626,11 +628,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     else if (NC >= 5) bp_rows_h<3>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
                     else bp_rows_h<2>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
                 } else {
-                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
+                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
+                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
+                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
+                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
                 }
                 if constexpr (ONE)
                     bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,

@@ -449,3 +449,48 @@ def test_compat_layer(ctx):
     assert v == pytest.approx(float(((proj.astype(np.float64) * 0.5) ** 2).sum()), rel=1e-6)
     np.testing.assert_allclose(a, (proj - b) ** 2, rtol=1e-6)
     lib.ct_compat_release()
+
+
+def test_volume_over_4gib_takes_64bit_path(ctx):
+    """A 1032 x 1032 x 1016 FP32 volume (4.3 GB, past the 32-bit buffer
+    offsets): the fast forward switches to 64-bit addressing with an explicit
+    range test; it must agree with the exact mode (the reference's traversal)
+    to the projection bar.  A small detector keeps the oracle out of it."""
+    g = Geometry(nx=1032, ny=1032, nz=1016, vx=0.25, ncols=24, nrows=16, du=3.0, nviews=3,
+                 dso=1000.0, dod=-500.0, angle_step_deg=31.0)
+    assert g.nvox * 4 > (1 << 32)
+    x = phantom(g, seed=5, scale=1.0, device="cuda")
+    fast = host(ctx.forward(g, x))
+    ctx.set_math("exact")
+    try:
+        exact = host(ctx.forward(g, x))
+    finally:
+        ctx.set_math("fast")
+    del x
+    torch.cuda.empty_cache()
+    r = rel(fast, exact)
+    print(f"4.3 GB volume: fast vs exact rel {r:.3e}")
+    assert np.abs(exact).max() > 0 and r <= 1e-5
+
+
+def test_empty_and_degenerate_inputs(ctx):
+    """Zero views are a no-op; a zero projection gives a zero adjoint and
+    leaves a SART update unchanged; rays that miss the volume give 0."""
+    g = GEOMS["small"]
+    x = dev(phantom(g, seed=3, scale=0.3).numpy())
+    before = x.clone()
+    lib, gc = core.lib(), g.to_c()
+    p = torch.full((1, *g.proj_shape[1:]), 7.0, device="cuda")
+    assert lib.ct_forward(ctx._h, ctypes.byref(gc), ctypes.c_void_p(x.data_ptr()), 0,
+                          ctypes.c_void_p(p.data_ptr()), 0, 3, 0) == 0
+    assert lib.ct_backproject_voxel(ctx._h, ctypes.byref(gc), ctypes.c_void_p(x.data_ptr()), 0,
+                                    ctypes.c_void_p(p.data_ptr()), 3, 0, ctypes.c_double(1.0), 1) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(x, before) and bool((p == 7.0).all())
+    z = torch.zeros((g.nviews, *g.proj_shape[1:]), device="cuda")
+    assert float(ctx.adjoint(g, z).abs().max()) == 0.0
+    ctx.backproject_voxel(g, x, z[:2], view0=0, nviews=2, beta=1.0, mode=1)
+    assert torch.equal(x, before)
+    # a detector far off to the side: every ray misses the volume
+    gm = small_geom(det_offset=500.0)
+    assert float(ctx.forward(gm, x).abs().max()) == 0.0
