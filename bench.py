@@ -257,18 +257,21 @@ def main() -> int:
                      "kernels_ms": {k: round(v[0], 3) for k, v in pf.items() if v[1]}}
     log(f"FDK (Hann, {g.nviews} views): {fdk_s:.3f} s")
     del vol_fdk
-    nsamp = 32
+    nbatch, bviews = 4, 8
+    nsamp = nbatch * bviews
     ctx.profile(True, reset=True)
-    for j in range(nsamp):
-        pj = ctx.forward(g, x32, view0=j * (g.nviews // nsamp), nviews=1)
-        ctx.adjoint(g, pj, view0=j * (g.nviews // nsamp), nviews=1)
+    for j in range(nbatch):
+        v0 = j * (g.nviews // nbatch)
+        pj = ctx.forward(g, x32, view0=v0, nviews=bviews)
+        ctx.adjoint(g, pj, view0=v0, nviews=bviews)
     torch.cuda.synchronize()
     pa = ctx.profile_read()
     t_pair = (pa["forward"][0] + pa["adjoint"][0]) / nsamp   # ms per view
     result["autograd_pair"] = {
         "grays": round(g.nrows * g.ncols / (t_pair * 1e-3) / 1e9, 4),
         "forward_ms": round(pa["forward"][0] / nsamp, 4), "adjoint_ms": round(pa["adjoint"][0] / nsamp, 4),
-        "sample": f"{nsamp} views spread over the orbit, ray sums then the matched adjoint (FP64 accumulation)"}
+        "sample": f"{nbatch} batches of {bviews} consecutive views spread over the orbit: ray sums, then the "
+                  "matched adjoint of the batch"}
     log(f"autograd pair: forward {pa['forward'][0] / nsamp:.3f} ms + adjoint {pa['adjoint'][0] / nsamp:.3f} ms per view")
     ctx.profile(False)
 

@@ -117,6 +117,11 @@ inline size_t dtype_size(ct_dtype d) {
     }
     return 0;
 }
+template <typename S> constexpr ct_dtype dtype_of();
+template <> constexpr ct_dtype dtype_of<float>() { return CT_F32; }
+template <> constexpr ct_dtype dtype_of<__half>() { return CT_F16; }
+template <> constexpr ct_dtype dtype_of<__hip_bfloat16>() { return CT_BF16; }
+template <> constexpr ct_dtype dtype_of<double>() { return CT_F64; }
 inline bool dtype_valid(int d) { return d == CT_F32 || d == CT_F16 || d == CT_BF16 || d == CT_F64; }
 
 // ------------------------------------------------------------ geometry

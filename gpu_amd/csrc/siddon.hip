@@ -542,24 +542,30 @@ __global__ __launch_bounds__(256) void k_forward(DGeo<T> d, const T *__restrict_
 // SART correction (CMD 1) of the sweep on a z-quad-layout volume (fast
 // mode, FP32 arithmetic, volume under 4 GiB): the same rays, segments and
 // output as k_forward<float, SV, float, SM, 1, true>.
-// CORR_SPLIT parts per ray: a workgroup holds 256 / CORR_SPLIT rays (a
-// 16x8-pixel tile for 2 parts, 8x8 for 4); part p walks alpha in
+// Projection on a z-quad-layout volume (fast mode, FP32 arithmetic, volume
+// under 4 GiB): CMD 0 ray sums, CMD 1 SART corrections (b - sum)/len, the same
+// rays and segments as k_forward.  S parts per ray: a workgroup holds 256 / S
+// rays (16x16 pixels for S = 1, 16x8 for 2, 8x8 for 4); part p walks alpha in
 // [lo + (hi - lo) p / S, lo + (hi - lo) (p + 1) / S] and the partial sums meet
-// in LDS, added in part order.  More waves per view and parts of equal length
-// keep more gathers in flight.
+// in LDS, added in part order.  Splitting gives more waves per view and parts
+// of equal length, so more gathers are in flight; it also splits the segment
+// that holds each cut point, so the SART correction uses it (S = 2) and the ray
+// sums of the autograd pair do not (their segments stay those of the matched
+// adjoint).
 constexpr int CORR_SPLIT = 2;
 
-template <typename SV, typename SM>
-__global__ __launch_bounds__(256) void k_correction_zq(DGeo<float> d, const float *__restrict__ ang,
-                                                       const SV *__restrict__ fq, float *__restrict__ out,
-                                                       const SM *__restrict__ meas, int first, int stride, int mv0,
-                                                       int z0) {
-    constexpr int S = CORR_SPLIT, NR = 256 / S;
-    __shared__ double part[256];
+template <typename SV, typename SO, typename SM, int CMD, int S>
+__global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *__restrict__ ang,
+                                                    const SV *__restrict__ fq, SO *__restrict__ out,
+                                                    const SM *__restrict__ meas, int first, int stride, int mv0,
+                                                    int z0) {
+    constexpr int NR = 256 / S, WPP = 4 / S;   // rays and waves per part
+    constexpr int TW = WPP >= 2 ? 16 : 8, TH = WPP == 4 ? 16 : 8;
+    __shared__ double part[S > 1 ? 256 : 1];
     const int tid = threadIdx.x, p = tid / NR, rr = tid % NR;
     const int w = rr >> 6, l = rr & 63;
     const int3 t = xcd_tile();
-    const int col = t.x * (S == 2 ? 16 : 8) + w * 8 + (l & 7), row = t.y * 8 + (l >> 3);
+    const int col = t.x * TW + (w & 1) * 8 + (l & 7), row = t.y * TH + (w >> 1) * 8 + (l >> 3);
     const int img = t.z + z0;
     const bool inb = col < d.R && row < d.ZP;
     const int q = img / d.NS, src = img - q * d.NS;
@@ -572,26 +578,31 @@ __global__ __launch_bounds__(256) void k_correction_zq(DGeo<float> d, const floa
         if (fw.init(d, s, c, col, row, src)) {
             len = fw.len;
             dist = fw.dist;
-            const float lo = fw.ac, span = fw.amax - fw.ac;
-            if (p + 1 < S) fw.amax = fmaf(span, (float)(p + 1) / (float)S, lo);
-            if (p > 0) fw.restart(d, fmaf(span, (float)p / (float)S, lo));
+            if constexpr (S > 1) {
+                const float lo = fw.ac, span = fw.amax - fw.ac;
+                if (p + 1 < S) fw.amax = fmaf(span, (float)(p + 1) / (float)S, lo);
+                if (p > 0) fw.restart(d, fmaf(span, (float)p / (float)S, lo));
+            }
             sum = walk_sum<SV, true, true>(fw, d, fq);
         }
     }
-    part[tid] = sum;
-    __syncthreads();
-    if (p != 0 || !inb) return;
+    if constexpr (S > 1) {
+        part[tid] = sum;
+        __syncthreads();
+        if (p != 0) return;
 #pragma unroll
-    for (int k = 1; k < S; ++k) sum += part[k * NR + rr];
+        for (int k = 1; k < S; ++k) sum += part[k * NR + rr];
+    }
+    if (!inb) return;
     const int64_t P = (int64_t)d.R * d.ZP;
     const int64_t pix = (int64_t)row * d.R + col;
     float r = 0.0f;
     if (!(len < d.vz * 1e-6f)) {
         const float ax = (float)(sum * (double)dist);
-        const float b = (float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix);
-        r = (b - ax) / len;
+        if constexpr (CMD == 0) r = ax;
+        else r = ((float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix) - ax) / len;
     }
-    out[(int64_t)img * P + pix] = r;
+    St<SO>::st(out, (int64_t)img * P + pix, r);
 }
 
 // Data fidelity sum (Ax - b)^2 fused into the ray traversal: Ax never goes to
@@ -1045,6 +1056,28 @@ static ct_status run_forward(ct_context *ctx, const ct_geometry &g, const void *
     // compensated FP32 ray sums in fast mode; the exact mode keeps the
     // reference's plain accumulation
     const bool comp = std::is_same<T, float>::value && ctx->math == CT_MATH_FAST;
+    if constexpr (std::is_same<T, float>::value) {
+        // four or more images: copy the volume once into the z-quad layout
+        // (coalesced gathers); ray sums keep one walk per ray, corrections
+        // split each ray in two
+        const uint64_t qbytes = (uint64_t)d.M * d.N * (uint64_t)((d.Z + 3) & ~3) * sizeof(SV);
+        if (comp && nimg >= 4 && qbytes < (1ull << 32)) {
+            void *volq = slot(ctx, SLOT_ZQVOL, qbytes, &st);
+            if (!volq) return st;
+            ProfScope ps(ctx, CT_K_FORWARD);   // the copy is part of the call's cost
+            st = launch_zq_pack(ctx, vol, dtype_of<SV>(), volq, d.M, d.N, d.Z);
+            if (st != CT_OK) return st;
+            constexpr int S = CMD == 1 ? CORR_SPLIT : 1;
+            constexpr int tw = S <= 2 ? 16 : 8, th = S == 1 ? 16 : 8;
+            for (int z0 = 0; z0 < nimg; z0 += 65535) {
+                const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
+                dim3 grid((d.R + tw - 1) / tw, (d.ZP + th - 1) / th, nz);
+                hipLaunchKernelGGL((k_project_zq<SV, SO, SM, CMD, S>), grid, dim3(256), 0, ctx->stream, d, ang,
+                                   (const SV *)volq, (SO *)out, (const SM *)meas, first, stride, mv0, z0);
+            }
+            return launch_check(ctx, "forward");
+        }
+    }
     ProfScope ps(ctx, CT_K_FORWARD);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
@@ -1120,10 +1153,10 @@ static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const 
     ProfScope ps(ctx, CT_K_FORWARD);
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
-        const int tw = CORR_SPLIT == 2 ? 16 : 8;
-        dim3 grid((d.R + tw - 1) / tw, (d.ZP + 7) / 8, nz);
-        hipLaunchKernelGGL((k_correction_zq<SV, SM>), grid, dim3(256), 0, ctx->stream, d, ang, (const SV *)volq, corr,
-                           (const SM *)meas, first, stride, 0, z0);
+        constexpr int tw = CORR_SPLIT <= 2 ? 16 : 8, th = CORR_SPLIT == 1 ? 16 : 8;
+        dim3 grid((d.R + tw - 1) / tw, (d.ZP + th - 1) / th, nz);
+        hipLaunchKernelGGL((k_project_zq<SV, float, SM, 1, CORR_SPLIT>), grid, dim3(256), 0, ctx->stream, d, ang,
+                           (const SV *)volq, corr, (const SM *)meas, first, stride, 0, z0);
     }
     return launch_check(ctx, "correction");
 }

@@ -350,8 +350,8 @@ __device__ __forceinline__ float dot2(unsigned a, unsigned b, float c) {
     return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c, false);
 }
 
-// rows of the packed-half engine: per row (A, B) = (kl/ez, kl*qz/ez) from
-// LDS, per column pair the staged (c*|S-P|, |S-P|) half pairs
+// rows of the packed-half engine: per row (qz, kl/ez) from LDS, per column
+// pair the staged (c*|S-P|, |S-P|) half pairs
 template <int NP>
 __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
                                           int cbase, float zl, float zh, float nA, const unsigned *lo2,
@@ -362,8 +362,9 @@ __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const fl
         uint2 pv[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) pv[p] = prow[2 * p];
-        const float t = nA - rt.y;
-        const unsigned az = pk_rtz(fmaf(zl, rt.x, t), fmaf(zh, rt.x, t));
+        // local crossings (z - qz)*(kl/ez) + nA; a horizontal ray (1/ez
+        // infinite) gives +-inf, or NaN exactly on a plane, as in FP32
+        const unsigned az = pk_rtz(fmaf(zl - rt.x, rt.y, nA), fmaf(zh - rt.x, rt.y, nA));
         const unsigned loz = pk_min_bc(az), hiz = pk_max_bc(az);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
@@ -392,7 +393,7 @@ __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const fl
 
 template <int NC>
 __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
-                                        int cbase, float zl, float vz, const float *lo, const float *hi, float &wk,
+                                        int cbase, float zl, float zh, const float *lo, const float *hi, float &wk,
                                         float &lk) {
     for (int j = z0; j <= z1; ++j) {
         const float2 rt = sr[j];
@@ -400,9 +401,11 @@ __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const flo
         float2 pv[NC];
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) pv[cc] = prow[cc];
-        // the two z-plane crossings of the row's ray; the upper one one voxel
-        // height further along
-        const float az0 = (zl - rt.x) * rt.y, az1 = fmaf(vz, rt.y, az0);
+        // the two z-plane crossings of the row's ray, in the reference's
+        // product form: a horizontal ray (1/ez infinite) gives +-inf, or NaN
+        // exactly on a plane, and min/max then resolve the slab as the
+        // reference does
+        const float az0 = (zl - rt.x) * rt.y, az1 = (zh - rt.x) * rt.y;
         const float loz = hmin(az0, az1), hiz = hmax(az0, az1);
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
@@ -447,7 +450,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                                                       int nv, float beta, int mode) {
     __shared__ float2 sp[LH * LP];   // (c*|S-P|, |S-P|); H: column-pair halves
     __shared__ float4 sc[LP];
-    __shared__ float2 sr[LH];        // (qz, 1/ez); H: (kl/ez, kl*qz/ez)
+    __shared__ float2 sr[LH];        // (qz, 1/ez); H: (qz, kl/ez)
     uint2 *const sph = reinterpret_cast<uint2 *>(sp);
     const int tid = threadIdx.y * BPX + threadIdx.x;
     const int3 tile = xcd_tile();   // each XCD: a contiguous z range of tiles
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             if (tid < LP) sc[tid] = tid < ncw ? cimg[C0 + tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (tid < nrow) {
                 const float2 rt = rimg[R0 + tid];
-                sr[tid] = H ? make_float2(kl * rt.y, kl * rt.x * rt.y) : rt;
+                sr[tid] = H ? make_float2(rt.x, kl * rt.y) : rt;
             }
             __syncthreads();
             if (!active || yoff) continue;
@@ -628,11 +631,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     else if (NC >= 5) bp_rows_h<3>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
                     else bp_rows_h<2>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
                 } else {
-                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
-                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
-                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
-                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
-                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, d.vz, lo, hi, wk, lk);
+                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                 }
                 if constexpr (ONE)
                     bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,

@@ -402,7 +402,7 @@ def test_reduced_precision_storage(ctx, dt):
 
 
 @pytest.mark.parametrize("dt,tol", [(torch.float16, 3e-3), (torch.bfloat16, 1.2e-2)])
-@pytest.mark.parametrize("name", ["small", "aniso", "multisource"])
+@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource"])
 @pytest.mark.parametrize("mode,nv", [(0, 1), (1, 1), (1, 3)])
 def test_backproject_reduced_precision(ctx, dt, tol, name, mode, nv):
     """FP16/BF16 volumes take the packed-half chord engine: chord intervals in
