@@ -230,7 +230,8 @@ enum Slot {
     SLOT_FFT = 8,      // twiddles
     SLOT_MISC = 9,
     SLOT_BPTAB = 10,   // per-view tables of the fast voxel-driven backprojector
-    SLOT_ZQVOL = 11    // the sweep's z-quad-layout volume
+    SLOT_ZQVOL = 11,   // the sweep's z-quad-layout volume
+    SLOT_FDKV = 12     // per-view constants of the fast FDK backprojector
 };
 
 ct_status fail(ct_context *ctx, ct_status s, const std::string &msg);

@@ -321,6 +321,275 @@ __global__ __launch_bounds__(256) void k_fdk_bp(DGeo<T> d, const T *__restrict__
     if constexpr (std::is_same<SV, double>::value) vol[idx] = acc; else St<SV>::st(vol, idx, (float)acc);
 }
 
+// Fast FP32 form (CT_MATH_FAST) of the same projection.  A lane walks a
+// column of voxels along z: for a fixed (x, y) and view, s, the magnification
+// weight (DSO/(DSO-s))^2 and the detector column p are shared by the whole
+// column and ksi is linear in z, so a voxel costs one FMA for ksi plus the
+// bilinear gather.  The reference's five IEEE divisions per voxel and view
+// (V63/FDK/kernel_FDK.cu:36-42) become one hardware reciprocal of (DSO - s)
+// per column and view: ksi = (z*(DSO-DOD)/(DSO-s) - zmin)/dv and
+// p = (t*(DSO-DOD)/(DSO-s) - ymin)/du with the DSO factors cancelled.  The
+// in-detector test is the reference's (the whole 2x2 stencil inside).
+// Per-view constants come from a table: cos and sin times (DSO-DOD)/du, then
+// sin and cos.
+__global__ void k_fdk_views(const float *__restrict__ ang, float4 *__restrict__ tab, int view0, int nv, float ku) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nv) return;
+    const float s = ang[2 * (view0 + q)], c = ang[2 * (view0 + q) + 1];
+    tab[q] = make_float4(c * ku, s * ku, s, c);
+}
+
+struct FdkCol {
+    float inv, wv, fb, bn0, k0, dk;
+};
+// Column constants for one view; wv is 0 for a column off the detector.
+__device__ __forceinline__ FdkCol fdk_column(const DGeo<float> &d, float4 vt, float x, float y, float wk, float pm,
+                                             float kz0, float kdz, float km) {
+    FdkCol r;
+    r.inv = frcp(fmaf(x, vt.z, fmaf(-y, vt.w, d.dso))); // 1/(DSO - s)
+    const float p = fmaf(fmaf(x, vt.x, y * vt.y), r.inv, pm);
+    r.wv = ((0.0f <= p) && (p + 1.0f < (float)d.R)) ? r.inv * r.inv * wk : 0.0f;
+    r.fb = floorf(p);
+    r.bn0 = p - r.fb;
+    r.k0 = fmaf(r.inv, kz0, km); // ksi = k0 + k*dk
+    r.dk = r.inv * kdz;
+    return r;
+}
+
+// Global-memory form for any geometry and view dtype: a lane owns FZC voxels
+// of a column; gathers are range-checked buffer loads with 32-bit offsets
+// into one view, indices clamped into the view and off-detector terms masked.
+constexpr int FZC = 8;
+template <typename SP, typename SV>
+__global__ __launch_bounds__(256) void k_fdk_bp_col(DGeo<float> d, const float4 *__restrict__ vtab,
+                                                    const SP *__restrict__ filt, SV *__restrict__ vol, int nv,
+                                                    float wscale) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y * 4 + threadIdx.y;
+    const int iz0 = blockIdx.z * FZC;
+    if (ix >= d.M || iy >= d.N) return;
+    const int nz = min(FZC, d.Z - iz0);
+    const float x = d.bx + d.vx * 0.5f + (float)ix * d.vx;
+    const float y = d.by + d.vy * 0.5f + (float)iy * d.vy;
+    const float z0 = d.bz + d.vz * 0.5f + (float)iz0 * d.vz;
+    const int64_t MN = (int64_t)d.M * d.N;
+    const int64_t idx0 = ((int64_t)iz0 * d.N + iy) * d.M + ix;
+    float acc[FZC];
+#pragma unroll
+    for (int k = 0; k < FZC; ++k) acc[k] = k < nz ? ldf(vol, idx0 + k * MN) : 0.0f;
+    const float sdd = d.dso - d.dod;
+    const int64_t P = (int64_t)d.R * d.ZP;
+    const uint32_t vbytes = (uint32_t)(P * (int64_t)sizeof(SP));
+    const uint32_t rowb = (uint32_t)d.R * (uint32_t)sizeof(SP);
+    const float wk = d.dso * d.dso * wscale, pm = -d.ymin * d.idu;
+    const float kz0 = z0 * sdd * d.idv, kdz = d.vz * sdd * d.idv, km = -d.zmin * d.idv;
+    const float zhi = (float)(d.ZP - 2), chi = (float)(d.R - 2);
+    for (int q = 0; q < nv; ++q) {
+        const FdkCol cc = fdk_column(d, vtab[q], x, y, wk, pm, kz0, kdz, km);
+        const uint32_t cb = (uint32_t)(int)__builtin_amdgcn_fmed3f(cc.fb, 0.0f, chi) * (uint32_t)sizeof(SP);
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(filt + (int64_t)q * P, vbytes);
+#pragma unroll
+        for (int k = 0; k < FZC; ++k) {
+            const float ksi = fmaf((float)k, cc.dk, cc.k0);
+            const float fa = floorf(ksi), an0 = ksi - fa;
+            const float fc = __builtin_amdgcn_fmed3f(fa, 0.0f, zhi);
+            const uint32_t off = (uint32_t)(int)fc * rowb + cb;
+            const float p00 = bload<SP>(rs, off), p01 = bload<SP>(rs, off + (uint32_t)sizeof(SP));
+            const float p10 = bload<SP>(rs, off + rowb), p11 = bload<SP>(rs, off + rowb + (uint32_t)sizeof(SP));
+            const float a0 = fmaf(cc.bn0, p01 - p00, p00), a1 = fmaf(cc.bn0, p11 - p10, p10);
+            acc[k] = fmaf(cc.wv, fc == fa ? fmaf(an0, a1 - a0, a0) : 0.0f, acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < FZC; ++k)
+        if (k < nz) St<SV>::st(vol, idx0 + k * MN, acc[k]);
+}
+
+// LDS-staged form for FP32 views.  A workgroup owns a tile of FTX x FTY
+// columns, FTZ voxels tall, and takes views FVB at a time: wave j bounds the
+// tile's detector footprint in view j by interval arithmetic on t, s and z
+// (a 0.5-pixel margin absorbs rounding), then the workgroup copies that
+// window into LDS, one 64-pixel row segment per wave instruction, by buffer
+// loads that write LDS directly.  Window rows sit FRS floats apart, so a
+// lane's 2x2 stencil is two ds_read2 with constant offsets; the stride of 80
+// spreads a wave's rows over the LDS banks, and 48 rows of it for two views
+// keep five workgroups per CU.  The host admits a geometry only when a bound
+// on every tile's window fits (fdk_lds_fits), so there is no slow path here;
+// the indices are still clamped into the window, which only moves reads of
+// voxels whose term the in-detector test zeroes.  Tiles clear of the
+// detector's top and bottom rows in every view skip the test and the clamp.
+constexpr int FTX = 16, FTY = 16, FTZ = 16, FVB = 2, FRS = 80, FROWS = 48;
+
+__device__ __forceinline__ void lds_row_dma(__amdgpu_buffer_rsrc_t rs, float *dst, uint32_t off) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)dst, 4, off, 0, 0, 0);
+}
+
+template <typename SV>
+__global__ __launch_bounds__(256) void k_fdk_bp_lds(DGeo<float> d, const float4 *__restrict__ vtab,
+                                                    const float *__restrict__ filt, SV *__restrict__ vol, int nv,
+                                                    float wscale) {
+    __shared__ float win[FVB][FROWS * FRS];
+    // per view of the batch: float bounds {first row, last stencil row, first
+    // column, last stencil column}; {first row, first column, rows, rows clear
+    // of both detector edges}
+    __shared__ float4 wlim[FVB];
+    __shared__ int4 wdim[FVB];
+    const int3 tile = xcd_tile();
+    const int tx = threadIdx.x & (FTX - 1), ty = threadIdx.x / FTX;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ix = tile.x * FTX + tx, iy = tile.y * FTY + ty, iz0 = tile.z * FTZ;
+    const bool live = ix < d.M && iy < d.N;
+    const int nz = min(FTZ, d.Z - iz0);
+    const float x = d.bx + d.vx * 0.5f + (float)ix * d.vx;
+    const float y = d.by + d.vy * 0.5f + (float)iy * d.vy;
+    const float z0 = d.bz + d.vz * 0.5f + (float)iz0 * d.vz;
+    // voxel-centre extent of the tile
+    const float xa = d.bx + d.vx * 0.5f + (float)(tile.x * FTX) * d.vx;
+    const float xb = d.bx + d.vx * 0.5f + (float)(min(tile.x * FTX + FTX, d.M) - 1) * d.vx;
+    const float ya = d.by + d.vy * 0.5f + (float)(tile.y * FTY) * d.vy;
+    const float yb = d.by + d.vy * 0.5f + (float)(min(tile.y * FTY + FTY, d.N) - 1) * d.vy;
+    const float zb = d.bz + d.vz * 0.5f + (float)(iz0 + nz - 1) * d.vz;
+    const float xm = 0.5f * (xa + xb), ym = 0.5f * (ya + yb), hx = 0.5f * (xb - xa), hy = 0.5f * (yb - ya);
+    const int64_t MN = (int64_t)d.M * d.N;
+    const int64_t idx0 = ((int64_t)iz0 * d.N + iy) * d.M + ix;
+    float acc[FTZ];
+#pragma unroll
+    for (int k = 0; k < FTZ; ++k) acc[k] = (live && k < nz) ? ldf(vol, idx0 + k * MN) : 0.0f;
+    const float sdd = d.dso - d.dod;
+    const int64_t P = (int64_t)d.R * d.ZP;
+    const uint32_t vbytes = (uint32_t)(P * (int64_t)sizeof(float));
+    const float wk = d.dso * d.dso * wscale, pm = -d.ymin * d.idu;
+    const float kz0 = z0 * sdd * d.idv, kdz = d.vz * sdd * d.idv, km = -d.zmin * d.idv;
+    // Tiles whose voxels stay clear of the detector's top and bottom rows in
+    // every view (for any s in [-r, r], r the tile's largest distance from
+    // the axis) skip the in-detector test; one code path per tile keeps the
+    // accumulators in place.
+    const float rt = sqrtf(fmaxf(xa * xa, xb * xb) + fmaxf(ya * ya, yb * yb));
+    const float ilo = frcp(d.dso + rt), ihi = frcp(d.dso - rt);
+    const float kall_lo = (fminf(z0 * ilo, z0 * ihi) * sdd - d.zmin) * d.idv;
+    const float kall_hi = (fmaxf(zb * ilo, zb * ihi) * sdd - d.zmin) * d.idv;
+    const bool clear = (d.dso - rt > 1e-3f * d.dso) && kall_lo >= 1.0f && kall_hi <= (float)(d.ZP - 3);
+    auto views = [&](auto tile_interior) {
+        for (int qb = 0; qb < nv; qb += FVB) {
+            const int nb = min(FVB, nv - qb);
+            __syncthreads(); // the previous batch's gathers are done
+            if (wave < nb) {
+                const float4 vt = vtab[qb + wave];
+                const float s = vt.z, c = vt.w;
+                const float tc = xm * c + ym * s, et = hx * fabsf(c) + hy * fabsf(s);
+                const float sc = -xm * s + ym * c, es = hx * fabsf(s) + hy * fabsf(c);
+                const float i0 = frcp(d.dso - (sc - es)), i1 = frcp(d.dso - (sc + es));
+                const float tlo = tc - et, thi = tc + et;
+                const float plo = (fminf(tlo * i0, tlo * i1) * sdd - d.ymin) * d.idu;
+                const float phi = (fmaxf(thi * i0, thi * i1) * sdd - d.ymin) * d.idu;
+                const float klo = (fminf(z0 * i0, z0 * i1) * sdd - d.zmin) * d.idv;
+                const float khi = (fmaxf(zb * i0, zb * i1) * sdd - d.zmin) * d.idv;
+                const float Rf = (float)d.R, Zf = (float)d.ZP;
+                const float fra = floorf(fminf(fmaxf(klo - 0.5f, -1.0f), Zf));
+                const float frb = floorf(fminf(fmaxf(khi + 0.5f, -1.0f), Zf)) + 1.0f;
+                const int ca = (int)floorf(fminf(fmaxf(plo - 0.5f, 0.0f), Rf));
+                const int cb = min(max((int)floorf(fminf(fmaxf(phi + 0.5f, 0.0f), Rf)) + 1, ca + 1), min(d.R - 1, ca + 63));
+                const int ra = max((int)fra, 0);
+                const int rb = min(max((int)frb, ra + 1), min(d.ZP - 1, ra + FROWS - 1));
+                if (lane == 0) {
+                    wdim[wave] = make_int4(ra, ca, rb - ra + 1, (fra >= 0.0f && frb <= (float)(d.ZP - 1)) ? 1 : 0);
+                    wlim[wave] = make_float4((float)ra, (float)(rb - 1), (float)ca, (float)(cb - 1));
+                }
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int j = 0; j < FVB; ++j) {
+                if (j >= nb) break;
+                const int4 dim = wdim[j];
+                const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(filt + (int64_t)(qb + j) * P, vbytes);
+                const uint32_t col = (uint32_t)min(dim.y + lane, d.R - 1);
+                const int rows = __builtin_amdgcn_readfirstlane(dim.z);
+                for (int r = wave; r < rows; r += 4)
+                    lds_row_dma(rs, &win[j][r * FRS], ((uint32_t)(dim.x + r) * (uint32_t)d.R + col) * 4u);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (!live) continue;
+    #pragma unroll
+            for (int j = 0; j < FVB; ++j) {
+                if (j >= nb) break;
+                const FdkCol cc = fdk_column(d, vtab[qb + j], x, y, wk, pm, kz0, kdz, km);
+                // Stencil rows clamp into [ra, rb-1], which lies inside [0, ZP-2]
+                // and holds the row of every voxel that passes the in-detector
+                // test, so "the clamp left the row unchanged" is that test
+                // (0 <= ksi < ZP-1); for a window clear of both detector edges
+                // every voxel of the tile passes it.
+                const int4 dim = wdim[j];
+                const float4 lim = wlim[j];
+                const float fbc = __builtin_amdgcn_fmed3f(cc.fb, lim.z, lim.w);
+                // byte offset of the window origin, view j's window base folded in
+                int base4 = ((int)fbc - dim.y - dim.x * FRS + j * (FROWS * FRS)) * 4;
+                asm("" : "+v"(base4)); // keep the window base inside the per-voxel address
+                const char *wj = (const char *)&win[0][0];
+                if constexpr (decltype(tile_interior)::value) {
+                    // every voxel of the tile passes the in-detector test in
+                    // every view, and the margin keeps its stencil in the window
+    #pragma unroll
+                    for (int k = 0; k < FTZ; ++k) {
+                        const float ksi = fmaf((float)k, cc.dk, cc.k0);
+                        const float fa = floorf(ksi), an0 = ksi - fa;
+                        const int o = __mul24((int)fa, FRS * 4) + base4;
+                        const float p00 = *(const float *)(wj + o), p01 = *(const float *)(wj + o + 4);
+                        const float p10 = *(const float *)(wj + o + FRS * 4), p11 = *(const float *)(wj + o + FRS * 4 + 4);
+                        const float a0 = fmaf(cc.bn0, p01 - p00, p00), a1 = fmaf(cc.bn0, p11 - p10, p10);
+                        acc[k] = fmaf(cc.wv, fmaf(an0, a1 - a0, a0), acc[k]);
+                    }
+                } else {
+    #pragma unroll
+                    for (int k = 0; k < FTZ; ++k) {
+                        const float ksi = fmaf((float)k, cc.dk, cc.k0);
+                        const float fa = floorf(ksi), an0 = ksi - fa;
+                        const float fc = __builtin_amdgcn_fmed3f(fa, lim.x, lim.y);
+                        const int o = __mul24((int)fc, FRS * 4) + base4;
+                        const float p00 = *(const float *)(wj + o), p01 = *(const float *)(wj + o + 4);
+                        const float p10 = *(const float *)(wj + o + FRS * 4), p11 = *(const float *)(wj + o + FRS * 4 + 4);
+                        const float a0 = fmaf(cc.bn0, p01 - p00, p00), a1 = fmaf(cc.bn0, p11 - p10, p10);
+                        acc[k] = fmaf(cc.wv, fc == fa ? fmaf(an0, a1 - a0, a0) : 0.0f, acc[k]);
+                    }
+                }
+            }
+        }
+    };
+    if (clear) views(std::true_type{});
+    else views(std::false_type{});
+    if (!live) return;
+#pragma unroll
+    for (int k = 0; k < FTZ; ++k)
+        if (k < nz) St<SV>::st(vol, idx0 + k * MN, acc[k]);
+}
+
+// Host bound on the LDS window of every tile and view.  A tile's voxel
+// centres lie within hd (the half-diagonal) of its centre, and DSO - s over
+// the tile lies in [D - hd, D + hd]; the window then spans at most
+// (2*hd/(D-hd) + (|t|+hd)*(1/(D-hd) - 1/(D+hd)))*(DSO-DOD)/du columns and
+// ((FTZ-1)*vz/(D-hd) + |z|max*(1/(D-hd) - 1/(D+hd)))*(DSO-DOD)/dv rows,
+// plus the margins.  Both grow with the distance r of the tile centre from
+// the axis, so the bound takes r at the grid corner and scans the angle.
+// False sends the call to k_fdk_bp_col.
+static bool fdk_lds_fits(const DGeo<float> &d) {
+    const double vx = d.vx, vy = d.vy, vz = d.vz, dso = d.dso, sdd = (double)d.dso - (double)d.dod;
+    const double x0 = d.bx + 0.5 * vx, x1 = d.bx + (d.M - 0.5) * vx;
+    const double y0 = d.by + 0.5 * vy, y1 = d.by + (d.N - 0.5) * vy;
+    const double rmax = std::sqrt(std::max(x0 * x0, x1 * x1) + std::max(y0 * y0, y1 * y1));
+    const double hd = 0.5 * std::sqrt(std::pow((FTX - 1) * vx, 2) + std::pow((FTY - 1) * vy, 2));
+    const double zmax = std::max(std::fabs(d.bz + 0.5 * vz), std::fabs(d.bz + (d.Z - 0.5) * vz));
+    if (!(dso - rmax - hd > 2e-3 * dso)) return false;
+    double cols = 0.0, rows = 0.0;
+    for (int i = 0; i <= 720; ++i) {
+        const double ph = 3.141592653589793 * i / 720.0;
+        const double t = rmax * std::fabs(std::cos(ph)), D = dso - rmax * std::sin(ph);
+        const double lo = D - hd, hi = D + hd, spread = 1.0 / lo - 1.0 / hi;
+        cols = std::max(cols, (2.0 * hd / lo + (t + hd) * spread) * sdd / d.du);
+        rows = std::max(rows, ((FTZ - 1) * vz / lo + zmax * spread) * sdd / d.dv);
+    }
+    return cols + 5.0 <= 64.0 && rows + 5.0 <= (double)FROWS;
+}
+
 template <typename T, typename SP, typename SV>
 static ct_status run_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, void *vol, int view0, int nv) {
     ct_status st;
@@ -332,6 +601,24 @@ static ct_status run_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *f
     const double w = (double)dth * 360.0 / (double)(d.us_rate * (T)g.nviews);
     dim3 block(64, 4, 1), grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
     ProfScope ps(ctx, CT_K_FDK_BP);
+    if constexpr (std::is_same<T, float>::value && !std::is_same<SP, double>::value) {
+        if (ctx->math == CT_MATH_FAST && (uint64_t)g.ncols * g.nrows * sizeof(SP) < (1ull << 32)) {
+            float4 *vt = (float4 *)slot(ctx, SLOT_FDKV, sizeof(float4) * (size_t)nv, &st);
+            if (!vt) return st;
+            hipLaunchKernelGGL(k_fdk_views, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const float *)ang, vt,
+                               view0, nv, (d.dso - d.dod) * d.idu);
+            if (std::is_same<SP, float>::value && fdk_lds_fits(d)) {
+                dim3 tg((d.M + FTX - 1) / FTX, (d.N + FTY - 1) / FTY, (d.Z + FTZ - 1) / FTZ);
+                hipLaunchKernelGGL((k_fdk_bp_lds<SV>), tg, dim3(256), 0, ctx->stream, d, (const float4 *)vt,
+                                   (const float *)filt, (SV *)vol, nv, (float)w);
+            } else {
+                dim3 cg((d.M + 63) / 64, (d.N + 3) / 4, (d.Z + FZC - 1) / FZC);
+                hipLaunchKernelGGL((k_fdk_bp_col<SP, SV>), cg, block, 0, ctx->stream, d, (const float4 *)vt,
+                                   (const SP *)filt, (SV *)vol, nv, (float)w);
+            }
+            return launch_check(ctx, "fdk backprojection");
+        }
+    }
     hipLaunchKernelGGL((k_fdk_bp<T, SP, SV>), grid, block, 0, ctx->stream, d, ang, (const SP *)filt, (SV *)vol, view0,
                        nv, (T)w);
     return launch_check(ctx, "fdk backprojection");
