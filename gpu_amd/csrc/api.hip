@@ -612,8 +612,7 @@ ct_status ct_fdk(ct_context *ctx, const ct_geometry *g, const void *proj, ct_dty
     ct_status st;
     void *w = slot(ctx, SLOT_FDK, dtype_size(fdt) * P, &st);
     if (!w) return st;
-    CT_TRY(launch_fdk_weight(ctx, *g, proj, pdt, w, fdt));
-    CT_TRY(launch_ramp(ctx, *g, w, fdt, w, fdt, g->nviews, window));
+    CT_TRY(launch_ramp(ctx, *g, proj, pdt, w, fdt, g->nviews, window, true)); // cosine weight + filter
     return launch_fdk_bp(ctx, *g, w, fdt, vol, vdt, 0, g->nviews);
 }
 

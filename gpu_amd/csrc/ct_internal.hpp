@@ -279,8 +279,9 @@ ct_status launch_fgp(ct_context *ctx, int M, int N, int Z, void *vol, ct_dtype d
 ct_status launch_fgp_D(ct_context *ctx, int M, int N, int Z, const void *x, void *r, void *s, void *t, ct_dtype dt);
 ct_status launch_fgp_Dt(ct_context *ctx, int M, int N, int Z, const void *r, const void *s, const void *t, void *out, ct_dtype dt);
 ct_status launch_fdk_weight(ct_context *ctx, const ct_geometry &g, const void *in, ct_dtype idt, void *out, ct_dtype odt);
+// weighted: apply the FDK cosine weight to the input rows first
 ct_status launch_ramp(ct_context *ctx, const ct_geometry &g, const void *in, ct_dtype idt, void *out, ct_dtype odt,
-                      int32_t nimg, ct_window window);
+                      int32_t nimg, ct_window window, bool weighted = false);
 ct_status launch_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, ct_dtype pdt, void *vol, ct_dtype vdt,
                         int32_t view0, int32_t nviews);
 

@@ -5,9 +5,11 @@
 // zero-padded to pad = 2*ncols, transformed, multiplied by the real even
 // filter H, transformed back and divided by pad.  H is real and even, so two
 // real rows a, b packed as a + i*b come back as a' + i*b': one complex
-// transform filters two rows.  One workgroup owns one row pair and keeps the
-// whole padded signal in LDS (radix-2 decimation in time with precomputed
-// FP64-accurate twiddles).  Non-power-of-two pads use a direct DFT.
+// transform filters two rows.  One workgroup owns one row pair.  Pads of
+// 2^9..2^12 run Stockham radix-8 passes in registers with LDS exchanges;
+// other powers of two keep the whole signal in LDS for radix-2 stages; both
+// use precomputed FP64-accurate twiddles.  Non-power-of-two pads use a
+// direct DFT.
 #include "ct_internal.hpp"
 #include <cmath>
 #include <type_traits>
@@ -115,6 +117,192 @@ __global__ __launch_bounds__(256) void k_ramp_fft(const SI *in, SO *out, int R, 
     }
 }
 
+// ------------------------------------------------ Stockham radix-8 filter
+// For pads of 2^9..2^12 the row-pair transform runs as Stockham passes
+// (Govindaraju et al., SC'08): a thread holds 8 complex values in registers
+// and does one radix-8 butterfly per pass (the last pass, radix 2^(LOGN-9),
+// does 8/R smaller ones), so a transform is three or four passes with an LDS
+// exchange between them instead of LOGN radix-2 stages.  Pass p with radix R
+// and sub-transform size Ns reads x[j + r*N/R], multiplies by
+// w^(r*(j mod Ns)*N/(Ns*R)), and writes x'[(j - j mod Ns)*R + j mod Ns + r*Ns];
+// input and output are in natural order.  The last forward pass leaves every
+// thread holding the spectrum at tid + r*N/8 up to a register permutation,
+// which is exactly the first inverse pass's input, so the filter multiply
+// needs no exchange; the inverse output is written straight to global memory.
+// With `weighted` the FDK cosine weight (V63/FDK/kernel_FDK.cu:8-13, the same
+// FP32 operations as k_fdk_weight) is applied as the rows are read.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); } // LDS bank spreading
+
+template <bool INV> __device__ __forceinline__ float2 twmul(float2 a, float2 w) {
+    // a * w forward, a * conj(w) inverse
+    return INV ? make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y)
+               : make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (-i) forward, a * (+i) inverse
+template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+template <bool INV> __device__ __forceinline__ void dft2(float2 *v) {
+    const float2 a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+}
+template <bool INV> __device__ __forceinline__ void dft4(float2 *v) {
+    const float2 c0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+    const float2 c1 = cadd(v[1], v[3]), d1 = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(c0, c1); v[2] = csub(c0, c1);
+    v[1] = cadd(d0, d1); v[3] = csub(d0, d1);
+}
+// X_2m = DFT4(x_n + x_n+4)_m, X_2m+1 = DFT4((x_n - x_n+4) w8^n)_m
+template <bool INV> __device__ __forceinline__ void dft8(float2 *v) {
+    constexpr float h = 0.70710678118654752f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) { a[n] = cadd(v[n], v[n + 4]); b[n] = csub(v[n], v[n + 4]); }
+    // b1 *= w8, b2 *= w8^2 = -i, b3 *= w8^3 (conjugates for the inverse)
+    b[1] = INV ? make_float2((b[1].x - b[1].y) * h, (b[1].x + b[1].y) * h)
+               : make_float2((b[1].x + b[1].y) * h, (b[1].y - b[1].x) * h);
+    b[2] = mul_mi<INV>(b[2]);
+    b[3] = INV ? make_float2(-(b[3].x + b[3].y) * h, (b[3].x - b[3].y) * h)
+               : make_float2((b[3].y - b[3].x) * h, -(b[3].x + b[3].y) * h);
+    dft4<INV>(a);
+    dft4<INV>(b);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { v[2 * m] = a[m]; v[2 * m + 1] = b[m]; }
+}
+template <int R, bool INV> __device__ __forceinline__ void dftr(float2 *v) {
+    if constexpr (R == 8) dft8<INV>(v);
+    else if constexpr (R == 4) dft4<INV>(v);
+    else dft2<INV>(v);
+}
+
+// one pass: twiddle and butterflies for the thread's 8/R butterflies
+// (butterfly b is j = tid + b*N/8); lns = log2(Ns)
+template <int LOGN, int R, bool INV>
+__device__ __forceinline__ void sh_pass(float2 (&v)[8], int tid, int lns, const float2 *tw) {
+    constexpr int N = 1 << LOGN, T = N / 8, NB = 8 / R, LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (lns > 0) {
+            const int j = tid + b * T;
+            const int k = j & ((1 << lns) - 1);
+            const int step = k << (LOGN - LR - lns); // k*N/(Ns*R)
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b * R + r] = twmul<INV>(v[b * R + r], tw[step * r]);
+        }
+        dftr<R, INV>(&v[b * R]);
+    }
+}
+// write the pass outputs (radix R, sub-transform size 2^lns) to LDS, then
+// read the next pass's inputs (radix RN)
+template <int LOGN, int R, int RN>
+__device__ __forceinline__ void sh_exchange(float2 (&v)[8], float2 *buf, int tid, int lns) {
+    constexpr int N = 1 << LOGN, T = N / 8;
+    __syncthreads(); // the previous pass's reads of buf are done
+#pragma unroll
+    for (int b = 0; b < 8 / R; ++b) {
+        const int j = tid + b * T;
+        const int k = j & ((1 << lns) - 1);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[lpad(base + (r << lns))] = v[b * R + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 8 / RN; ++b) {
+        const int j = tid + b * T;
+#pragma unroll
+        for (int r = 0; r < RN; ++r) v[b * RN + r] = buf[lpad(j + r * (N / RN))];
+    }
+}
+// full transform; v holds x[tid + r*N/8] on entry and X[tid + perm(r)*N/8]
+// on exit, with perm(b*RL + r) = b + r*(8/RL)
+template <int LOGN, bool INV>
+__device__ __forceinline__ void sh_fft(float2 (&v)[8], float2 *buf, int tid, const float2 *tw) {
+    sh_pass<LOGN, 8, INV>(v, tid, 0, tw);
+    sh_exchange<LOGN, 8, 8>(v, buf, tid, 0);
+    sh_pass<LOGN, 8, INV>(v, tid, 3, tw);
+    sh_exchange<LOGN, 8, 8>(v, buf, tid, 3);
+    sh_pass<LOGN, 8, INV>(v, tid, 6, tw);
+    if constexpr (LOGN > 9) {
+        constexpr int RL = 1 << (LOGN - 9);
+        sh_exchange<LOGN, 8, RL>(v, buf, tid, 6);
+        sh_pass<LOGN, RL, INV>(v, tid, 9, tw);
+    }
+}
+template <int LOGN> __device__ __forceinline__ constexpr int sh_perm(int i) {
+    constexpr int RL = LOGN > 9 ? 1 << (LOGN - 9) : 8;
+    return (i / RL) + (i % RL) * (8 / RL);
+}
+
+template <typename SI>
+__device__ __forceinline__ float fdk_weighted(const DGeo<float> &d, const SI *in, int64_t idx, int x, int z) {
+    const float x1 = d.dso;
+    const float x2 = (d.ymin + (float)x * d.du) * d.dso / (d.dso - d.dod);
+    const float x3 = (d.zmin + (float)z * d.dv) * d.dso / (d.dso - d.dod);
+    const float a = ldx(in, idx);
+    return a * d.dso / sqrtf(x1 * x1 + x2 * x2 + x3 * x3);
+}
+
+template <int LOGN, typename SI, typename SO, bool WEIGHT>
+__global__ __launch_bounds__(1 << (LOGN - 3)) void k_ramp_sh(const SI *__restrict__ in, SO *__restrict__ out, int R,
+                                                            int64_t nrows_total, const float2 *__restrict__ tw,
+                                                            const float *__restrict__ H, DGeo<float> d) {
+    constexpr int N = 1 << LOGN, T = N / 8;
+    __shared__ float2 buf[N + N / 16];
+    __shared__ float2 stw[N];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) stw[tid + r * T] = tw[tid + r * T];
+    const int64_t ra = (int64_t)blockIdx.x * 2, rb = ra + 1;
+    const bool hb = rb < nrows_total;
+    const int za = (int)(ra % d.ZP), zb = (int)(rb % d.ZP);
+    float2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int n = tid + r * T;
+        float a = 0.0f, b = 0.0f;
+        if (n < R) {
+            if constexpr (WEIGHT) {
+                a = fdk_weighted(d, in, ra * R + n, n, za);
+                if (hb) b = fdk_weighted(d, in, rb * R + n, n, zb);
+            } else {
+                a = ldx(in, ra * R + n);
+                if (hb) b = ldx(in, rb * R + n);
+            }
+        }
+        v[r] = make_float2(a, b);
+    }
+    __syncthreads();
+    sh_fft<LOGN, false>(v, buf, tid, stw);
+    // H is real and even: filtering a + ib filters both rows
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float h = H[tid + sh_perm<LOGN>(i) * T];
+        v[i].x *= h; v[i].y *= h;
+    }
+    float2 u[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) u[sh_perm<LOGN>(i)] = v[i];
+    sh_fft<LOGN, true>(u, buf, tid, stw);
+    const float inv = 1.0f / (float)N;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int n = tid + sh_perm<LOGN>(i) * T;
+        if (n < R) {
+            if constexpr (std::is_same<SO, double>::value) {
+                out[ra * R + n] = (double)(u[i].x * inv);
+                if (hb) out[rb * R + n] = (double)(u[i].y * inv);
+            } else {
+                St<SO>::st(out, ra * R + n, u[i].x * inv);
+                if (hb) St<SO>::st(out, rb * R + n, u[i].y * inv);
+            }
+        }
+    }
+}
+
 // Direct DFT for pads that are not powers of two; tw holds n entries.
 template <typename SI, typename SO>
 __global__ __launch_bounds__(256) void k_ramp_dft(const SI *in, SO *out, int R, int64_t nrows_total, int pad,
@@ -160,7 +348,7 @@ __global__ __launch_bounds__(256) void k_ramp_dft(const SI *in, SO *out, int R, 
 
 // H of V63/FDK/FFT_1d_GPU.h:243-269 (computed in float as the reference does),
 // optionally Hann-windowed; twiddles in FP64 rounded to float.
-static void filter_tables(const ct_geometry &g, ct_window window, bool pow2, std::vector<float2> &tw,
+static void filter_tables(const ct_geometry &g, ct_window window, std::vector<float2> &tw,
                           std::vector<float> &H) {
     const int pad = 2 * g.ncols;
     const float du = (float)g.du, dso = (float)g.dso, dod = (float)g.dod;
@@ -175,9 +363,8 @@ static void filter_tables(const ct_geometry &g, ct_window window, bool pow2, std
         if (window == CT_WINDOW_HANN) hv *= 0.5 * (1.0 + std::cos(2.0 * M_PI * (double)k / (double)pad));
         H[i] = (float)hv;
     }
-    const int nt = pow2 ? pad / 2 : pad;
-    tw.resize(nt);
-    for (int k = 0; k < nt; ++k) {
+    tw.resize(pad);
+    for (int k = 0; k < pad; ++k) {
         const double a = -2.0 * M_PI * (double)k / (double)pad;
         tw[k].x = (float)std::cos(a);
         tw[k].y = (float)std::sin(a);
@@ -185,16 +372,39 @@ static void filter_tables(const ct_geometry &g, ct_window window, bool pow2, std
 }
 
 template <typename SI, typename SO>
-static ct_status run_ramp(ct_context *ctx, const ct_geometry &g, const void *in, void *out, int nimg, ct_window window) {
+static ct_status run_weight(ct_context *ctx, const ct_geometry &g, const void *in, void *out);
+
+template <int LOGN, typename SI, typename SO>
+static void launch_sh(ct_context *ctx, const ct_geometry &g, const void *in, void *out, int64_t rows, int64_t pairs,
+                      const float2 *tw, const float *H, bool weighted) {
+    const DGeo<float> d = derive<float>(g);
+    if (weighted)
+        hipLaunchKernelGGL((k_ramp_sh<LOGN, SI, SO, true>), dim3((unsigned)pairs), dim3(1 << (LOGN - 3)), 0,
+                           ctx->stream, (const SI *)in, (SO *)out, g.ncols, rows, tw, H, d);
+    else
+        hipLaunchKernelGGL((k_ramp_sh<LOGN, SI, SO, false>), dim3((unsigned)pairs), dim3(1 << (LOGN - 3)), 0,
+                           ctx->stream, (const SI *)in, (SO *)out, g.ncols, rows, tw, H, d);
+}
+
+template <typename SI, typename SO>
+static ct_status run_ramp(ct_context *ctx, const ct_geometry &g, const void *in, void *out, int nimg, ct_window window,
+                          bool weighted) {
     const int R = g.ncols, pad = 2 * R;
     const bool pow2 = (pad & (pad - 1)) == 0;
     int logn = 0;
     while ((1 << logn) < pad) ++logn;
+    const bool sh = pow2 && logn >= 9 && logn <= 12;
     const size_t lds = sizeof(cplx) * (pow2 ? pad : 2 * pad);
-    if (lds > 64 * 1024) return fail(ctx, CT_ERR_INVALID_ARG, "ramp filter: padded row does not fit in LDS");
+    if (!sh && lds > 64 * 1024) return fail(ctx, CT_ERR_INVALID_ARG, "ramp filter: padded row does not fit in LDS");
+    if (weighted && (!sh || std::is_same<SI, double>::value || std::is_same<SO, double>::value)) {
+        // cosine weight as its own pass, then filter in place
+        const ct_status st = run_weight<SI, SO>(ctx, g, in, out);
+        if (st != CT_OK) return st;
+        return run_ramp<SO, SO>(ctx, g, out, out, nimg, window, false);
+    }
     std::vector<float2> tw;
     std::vector<float> H;
-    filter_tables(g, window, pow2, tw, H);
+    filter_tables(g, window, tw, H);
     ct_status st;
     char *tab = (char *)slot(ctx, SLOT_FFT, sizeof(float2) * tw.size() + sizeof(float) * H.size(), &st);
     if (!tab) return st;
@@ -211,7 +421,14 @@ static ct_status run_ramp(ct_context *ctx, const ct_geometry &g, const void *in,
     const int64_t pairs = (rows + 1) / 2;
     if (pairs > 2147483647LL) return fail(ctx, CT_ERR_OVERFLOW, "ramp filter: too many rows");
     ProfScope ps(ctx, CT_K_RAMP);
-    if (pow2)
+    if (sh) {
+        switch (logn) {
+        case 9: launch_sh<9, SI, SO>(ctx, g, in, out, rows, pairs, dtw, dH, weighted); break;
+        case 10: launch_sh<10, SI, SO>(ctx, g, in, out, rows, pairs, dtw, dH, weighted); break;
+        case 11: launch_sh<11, SI, SO>(ctx, g, in, out, rows, pairs, dtw, dH, weighted); break;
+        default: launch_sh<12, SI, SO>(ctx, g, in, out, rows, pairs, dtw, dH, weighted); break;
+        }
+    } else if (pow2)
         hipLaunchKernelGGL((k_ramp_fft<SI, SO>), dim3((unsigned)pairs), dim3(256), lds, ctx->stream, (const SI *)in,
                            (SO *)out, R, rows, pad, logn, (const float2 *)dtw, (const float *)dH);
     else
@@ -222,24 +439,24 @@ static ct_status run_ramp(ct_context *ctx, const ct_geometry &g, const void *in,
 
 template <typename SI>
 static ct_status ramp_out(ct_context *ctx, const ct_geometry &g, const void *in, void *out, ct_dtype odt, int nimg,
-                          ct_window w) {
+                          ct_window w, bool weighted) {
     switch (odt) {
-    case CT_F32: return run_ramp<SI, float>(ctx, g, in, out, nimg, w);
-    case CT_F16: return run_ramp<SI, __half>(ctx, g, in, out, nimg, w);
-    case CT_BF16: return run_ramp<SI, __hip_bfloat16>(ctx, g, in, out, nimg, w);
-    case CT_F64: return run_ramp<SI, double>(ctx, g, in, out, nimg, w);
+    case CT_F32: return run_ramp<SI, float>(ctx, g, in, out, nimg, w, weighted);
+    case CT_F16: return run_ramp<SI, __half>(ctx, g, in, out, nimg, w, weighted);
+    case CT_BF16: return run_ramp<SI, __hip_bfloat16>(ctx, g, in, out, nimg, w, weighted);
+    case CT_F64: return run_ramp<SI, double>(ctx, g, in, out, nimg, w, weighted);
     }
     return fail(ctx, CT_ERR_DTYPE, "ramp: bad output dtype");
 }
 
 ct_status launch_ramp(ct_context *ctx, const ct_geometry &g, const void *in, ct_dtype idt, void *out, ct_dtype odt,
-                      int32_t nimg, ct_window window) {
+                      int32_t nimg, ct_window window, bool weighted) {
     if (nimg <= 0) return CT_OK;
     switch (idt) {
-    case CT_F32: return ramp_out<float>(ctx, g, in, out, odt, nimg, window);
-    case CT_F16: return ramp_out<__half>(ctx, g, in, out, odt, nimg, window);
-    case CT_BF16: return ramp_out<__hip_bfloat16>(ctx, g, in, out, odt, nimg, window);
-    case CT_F64: return ramp_out<double>(ctx, g, in, out, odt, nimg, window);
+    case CT_F32: return ramp_out<float>(ctx, g, in, out, odt, nimg, window, weighted);
+    case CT_F16: return ramp_out<__half>(ctx, g, in, out, odt, nimg, window, weighted);
+    case CT_BF16: return ramp_out<__hip_bfloat16>(ctx, g, in, out, odt, nimg, window, weighted);
+    case CT_F64: return ramp_out<double>(ctx, g, in, out, odt, nimg, window, weighted);
     }
     return fail(ctx, CT_ERR_DTYPE, "ramp: bad input dtype");
 }

@@ -410,6 +410,35 @@ def test_ramp_filter_non_power_of_two(ctx):
     assert rel(got, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("ncols", [8, 256, 512, 1024, 2048])
+def test_ramp_filter_power_of_two_pads(ctx, ncols):
+    """Pads 2^9..2^12 take the Stockham radix-8 kernel, 16 the radix-2 LDS
+    kernel; an odd number of rows leaves the last pair with one row.  Against
+    the FP64 restatement (numpy-pinned) to 1e-5, both windows."""
+    g = Geometry(nx=8, ny=8, nz=8, vx=1.0, ncols=ncols, nrows=3, du=0.9, nviews=3, dso=100.0, dod=-50.0)
+    p = np.random.default_rng(ncols).standard_normal((3, 3, ncols)).astype(np.float32)
+    for window in ("ramp", "hann"):
+        ref = orc.ramp_filter(g, p, window, prec=64)
+        got = host(ctx.ramp_filter(g, dev(p), window=window))
+        r = rel(got, ref)
+        print(f"ncols {ncols} {window}: rel {r:.3e}")
+        assert r <= 1e-5
+
+
+def test_fdk_fused_weight_and_filter(ctx):
+    """ct_fdk applies the cosine weight while the Stockham filter reads the
+    rows (pad 512); the result matches the FP32 restatement of weight, filter
+    and backprojection."""
+    g = Geometry(nx=32, ny=32, nz=16, vx=1.0, ncols=256, nrows=24, du=0.5, nviews=20, dso=300.0, dod=-150.0)
+    x = phantom(g, seed=9, scale=0.4).numpy()
+    p = orc.forward(g, x, prec=32)
+    ref = orc.fdk(g, p, "hann", prec=32)
+    got = host(ctx.fdk(g, dev(p), window="hann"))
+    r = rel(got, ref)
+    print(f"fused FDK: rel {r:.3e}")
+    assert r <= 1e-4
+
+
 def test_sart_fista_iteration_matches_composition(ctx):
     g = GEOMS["small"]
     x = phantom(g, seed=16, scale=0.3).numpy()
