@@ -80,6 +80,16 @@ def _load() -> ctypes.CDLL:
         "ct_profile_enable": ([P, I], I),
         "ct_profile_reset": ([P], I),
         "ct_profile_read": ([P, I, DP, ctypes.POINTER(ctypes.c_int64)], I),
+        "ct_projection_path": ([G, I, I, ctypes.c_char_p, ctypes.c_char_p, I], I),
+        "ct_read_projections": ([P, G, ctypes.c_char_p, I, I, P, I, I, I], I),
+        "ct_log_transform": ([P, P, I, P, P, D, P, I, I64, I, D], I),
+        "ct_write_volume": ([P, G, P, I, ctypes.c_char_p], I),
+        "ct_read_volume": ([P, G, ctypes.c_char_p, P, I], I),
+        "ct_recon_path": ([G, D, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, I, D, D, D, ctypes.c_char_p, I], I),
+        "ct_objective_path": ([G, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, D, D, ctypes.c_char_p, I], I),
+        "ct_write_objective": ([ctypes.c_char_p, DP, I64], I),
+        "ct_save_checkpoint": ([P, G, ctypes.c_char_p, P, P, I, D, I], I),
+        "ct_load_checkpoint": ([P, G, ctypes.c_char_p, P, P, I, DP, ctypes.POINTER(ctypes.c_int32)], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -376,6 +386,66 @@ class Context:
         self._check(_lib.ct_convert(self._h, _ptr(src), dtype_code(src), _ptr(dst), dtype_code(dst), src.numel()))
         return dst
 
+    # ---- files (main_recon.cu:196-224 projections in, :437-487 outputs)
+    def read_projections(self, g: Geometry, directory: str, proj: torch.Tensor | None = None, view0: int = 0,
+                         nviews: int | None = None, row0: int = 0, tomo: bool = False,
+                         dtype: torch.dtype = torch.float32) -> torch.Tensor:
+        nv = g.nviews - view0 if nviews is None else nviews
+        if proj is None:
+            proj = torch.empty((nv, g.nrows, g.ncols), device="cuda", dtype=dtype)
+        _need(proj, nv * g.nrows * g.ncols, "proj")
+        self._stream()
+        gc = g.to_c()
+        self._check(_lib.ct_read_projections(self._h, ctypes.byref(gc), directory.encode(), int(tomo), row0,
+                                             _ptr(proj), dtype_code(proj), view0, nv))
+        return proj
+
+    def log_transform(self, raw: torch.Tensor, flat: torch.Tensor | None = None, dark: torch.Tensor | None = None,
+                      i0: float = 1.0, out: torch.Tensor | None = None, floor: float = 1e-6) -> torch.Tensor:
+        """-ln(max((raw - dark)/(flat - dark), floor)); flat and dark are one image, broadcast over raw's
+        leading dimension (flat None: the scalar i0, dark None: 0)."""
+        pix = raw.shape[-2] * raw.shape[-1]
+        out = torch.empty_like(raw, dtype=torch.float32) if out is None else out
+        for t, name in ((flat, "flat"), (dark, "dark")):
+            if t is not None:
+                if t.dtype != torch.float32:
+                    raise CtError(3, f"{name} must be float32")
+                _need(t, pix, name)
+        self._stream()
+        self._check(_lib.ct_log_transform(self._h, _ptr(raw), dtype_code(raw), _ptr(flat), _ptr(dark), i0, _ptr(out),
+                                          dtype_code(out), pix, raw.numel() // pix, floor))
+        return out
+
+    def write_volume(self, g: Geometry, vol: torch.Tensor, path: str) -> None:
+        _need(vol, g.nx * g.ny * g.nz, "vol")
+        self._stream()
+        gc = g.to_c()
+        self._check(_lib.ct_write_volume(self._h, ctypes.byref(gc), _ptr(vol), dtype_code(vol), str(path).encode()))
+
+    def read_volume(self, g: Geometry, path: str, vol: torch.Tensor | None = None,
+                    dtype: torch.dtype = torch.float32) -> torch.Tensor:
+        vol = torch.empty(g.vol_shape, device="cuda", dtype=dtype) if vol is None else vol
+        _need(vol, g.nx * g.ny * g.nz, "vol")
+        self._stream()
+        gc = g.to_c()
+        self._check(_lib.ct_read_volume(self._h, ctypes.byref(gc), str(path).encode(), _ptr(vol), dtype_code(vol)))
+        return vol
+
+    def save_checkpoint(self, g: Geometry, path: str, x: torch.Tensor, x_prev: torch.Tensor, t_state: float,
+                        iteration: int) -> None:
+        self._stream()
+        gc = g.to_c()
+        self._check(_lib.ct_save_checkpoint(self._h, ctypes.byref(gc), str(path).encode(), _ptr(x), _ptr(x_prev),
+                                            dtype_code(x), t_state, iteration))
+
+    def load_checkpoint(self, g: Geometry, path: str, x: torch.Tensor, x_prev: torch.Tensor) -> tuple[float, int]:
+        t, k = ctypes.c_double(), ctypes.c_int32()
+        self._stream()
+        gc = g.to_c()
+        self._check(_lib.ct_load_checkpoint(self._h, ctypes.byref(gc), str(path).encode(), _ptr(x), _ptr(x_prev),
+                                            dtype_code(x), ctypes.byref(t), ctypes.byref(k)))
+        return t.value, k.value
+
 
 def validate_geometry(g: Geometry) -> int:
     gc = g.to_c()
@@ -389,3 +459,42 @@ def view_angle(g: Geometry, view: int, fdk: bool = False) -> tuple[float, float]
     if st != 0:
         raise CtError(st, _lib.ct_status_string(st).decode())
     return s.value, c.value
+
+
+def _path(st: int, buf) -> str:
+    if st != 0:
+        raise CtError(st, _lib.ct_status_string(st).decode())
+    return buf.value.decode()
+
+
+def projection_path(g: Geometry, view: int, directory: str, tomo: bool = False) -> str:
+    """The reference's per-view file name (main_recon.cu:199-224)."""
+    buf = ctypes.create_string_buffer(4096)
+    gc = g.to_c()
+    return _path(_lib.ct_projection_path(ctypes.byref(gc), view, int(tomo), directory.encode(), buf, 4096), buf)
+
+
+def recon_path(g: Geometry, directory: str, output_folder: str, object_name: str, iteration: int = -1,
+               lambda_tv: float = 0.0, lambda_l1: float = 0.0, lipschitz: float = 32.0, unit_um: float = 1e3) -> str:
+    """The reference's .recon name (main_recon.cu:438,452): iteration < 0 gives the running file."""
+    buf = ctypes.create_string_buffer(4096)
+    gc = g.to_c()
+    return _path(_lib.ct_recon_path(ctypes.byref(gc), unit_um, directory.encode(), output_folder.encode(),
+                                    object_name.encode(), iteration, lambda_tv, lambda_l1, lipschitz, buf, 4096), buf)
+
+
+def objective_path(g: Geometry, directory: str, output_folder: str, object_name: str, lambda_tv: float = 0.0,
+                   lipschitz: float = 32.0) -> str:
+    """The reference's object_func file name (main_recon.cu:468)."""
+    buf = ctypes.create_string_buffer(4096)
+    gc = g.to_c()
+    return _path(_lib.ct_objective_path(ctypes.byref(gc), directory.encode(), output_folder.encode(),
+                                        object_name.encode(), lambda_tv, lipschitz, buf, 4096), buf)
+
+
+def write_objective(path: str, values) -> None:
+    """(tv, data fidelity, objective) triples as doubles (main_recon.cu:467-477)."""
+    arr = (ctypes.c_double * len(values))(*[float(v) for v in values])
+    st = _lib.ct_write_objective(str(path).encode(), arr, len(values))
+    if st != 0:
+        raise CtError(st, _lib.ct_status_string(st).decode())

@@ -158,9 +158,7 @@ static ct_status dims_ok(ct_context *ctx, int32_t nx, int32_t ny, int32_t nz) {
     return CT_OK;
 }
 
-struct DeviceGuard {
-    explicit DeviceGuard(ct_context *ctx) { hipSetDevice(ctx->device); }
-};
+ct_status validate_geometry(ct_context *ctx, const ct_geometry *g) { return validate(ctx, g); }
 
 static int64_t vox(const ct_geometry *g) { return (int64_t)g->nx * g->ny * g->nz; }
 static int64_t pix(const ct_geometry *g) { return (int64_t)g->ncols * g->nrows * (g->nsources < 1 ? 1 : g->nsources); }
@@ -241,6 +239,7 @@ const char *ct_status_string(ct_status s) {
     case CT_ERR_ALLOC: return "allocation failed";
     case CT_ERR_DEVICE: return "device error";
     case CT_ERR_LAUNCH: return "launch failed";
+    case CT_ERR_IO: return "file error";
     }
     return "unknown status";
 }

@@ -283,6 +283,24 @@ double TV_norm_gpu(float *d_volume) {
     return r;
 }
 
+void SaveDeviceDataToFile(float *d_data, int data_size, char filename[]) {
+    std::lock_guard<std::mutex> l(mtx());
+    if (!d_data || !filename || data_size < 0) { set_err(CT_ERR_NULL, "SaveDeviceDataToFile: bad arguments"); return; }
+    const size_t bytes = sizeof(float) * (size_t)data_size;
+    std::vector<float> host;
+    const float *src = d_data;
+    if (is_device(d_data)) {
+        host.resize((size_t)data_size);
+        if (!download(host.data(), d_data, bytes)) return;
+        src = host.data();
+    }
+    std::FILE *fp = std::fopen(filename, "wb");
+    if (!fp) { set_err(CT_ERR_IO, std::string("can not open file to write data ") + filename); return; }
+    const bool w = std::fwrite(src, 1, bytes, fp) == bytes;
+    if (std::fclose(fp) != 0 || !w) { set_err(CT_ERR_IO, std::string("write failed: ") + filename); return; }
+    set_err(0, "");
+}
+
 double L2_norm_cpu(const float *a, const float *b, int SIZE) {
     double s = 0;
     for (int i = 0; i < SIZE; ++i) { double e = (double)a[i] - (double)b[i]; s += e * e; }

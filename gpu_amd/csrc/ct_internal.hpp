@@ -231,7 +231,8 @@ enum Slot {
     SLOT_MISC = 9,
     SLOT_BPTAB = 10,   // per-view tables of the fast voxel-driven backprojector
     SLOT_ZQVOL = 11,   // the sweep's z-quad-layout volume
-    SLOT_FDKV = 12     // per-view constants of the fast FDK backprojector
+    SLOT_FDKV = 12,    // per-view constants of the fast FDK backprojector
+    SLOT_IO = 13       // FP32 staging of file reads and writes
 };
 
 ct_status fail(ct_context *ctx, ct_status s, const std::string &msg);
@@ -241,6 +242,11 @@ void *slot(ct_context *ctx, int which, size_t bytes, ct_status *st);
 // Device angle table (T = float or double), sin at [2v], cos at [2v+1].
 const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fdk, ct_status *st);
 ct_status check_ptr(ct_context *ctx, const void *p, ct_dtype dt, const char *name);
+ct_status validate_geometry(ct_context *ctx, const ct_geometry *g);
+// makes the context's device current for the calls of one entry point
+struct DeviceGuard {
+    explicit DeviceGuard(ct_context *ctx) { hipSetDevice(ctx->device); }
+};
 
 // launchers implemented per translation unit
 ct_status launch_forward(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt,

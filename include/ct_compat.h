@@ -47,6 +47,9 @@ void RampFiltering_GPU(float *g_in, float *g_out);
 double L2_norm_gpu(float *d_a, float *d_b);
 /* V63/host_tool_functions.cu:61 */
 double TV_norm_gpu(float *d_volume);
+/* V63/host_tool_functions.cu:2 - writes data_size floats (host or device)
+ * to a raw file; a failure sets the compat status instead of exiting. */
+void SaveDeviceDataToFile(float *d_data, int data_size, char filename[]);
 /* V63/host_tool_functions.cu:18 (host arithmetic, FP64 sum) */
 double L2_norm_cpu(const float *a, const float *b, int SIZE);
 /* V63/host_tool_functions.cu:105 (host arithmetic, FP64 sum) */

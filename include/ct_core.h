@@ -46,7 +46,8 @@ typedef enum ct_status {
     CT_ERR_OVERFLOW = 6,      /* a size exceeds the kernels' 32-bit index range */
     CT_ERR_ALLOC = 7,         /* device scratch allocation failed */
     CT_ERR_DEVICE = 8,        /* no usable HIP device / HIP runtime error */
-    CT_ERR_LAUNCH = 9         /* kernel launch failed */
+    CT_ERR_LAUNCH = 9,        /* kernel launch failed */
+    CT_ERR_IO = 10            /* a file could not be opened, read or written, or is malformed */
 } ct_status;
 
 /* Storage types.  Arithmetic is FP32 for F32/F16/BF16 storage and FP64 for
@@ -305,6 +306,77 @@ typedef enum ct_kernel_id {
 ct_status ct_profile_enable(ct_context *ctx, int32_t enable);
 ct_status ct_profile_reset(ct_context *ctx);
 ct_status ct_profile_read(ct_context *ctx, ct_kernel_id kernel, double *total_ms, int64_t *launches);
+
+/* ---- files: projections in, volumes / objective / checkpoints out ------- */
+/* These calls move data between files and device buffers: they stage through
+ * pinned host memory and return once the data is in place (synchronous). */
+
+/* Path of view `view`'s raw float32 projection file: directory +
+ * "Projections/View_%.02f.dat" with the reference's file angle
+ * float(view*angle_step + angle0), the second half of the views at
+ * 180 + (view - nviews/2)*angle_step + angle0 when tomo != 0, and +360 for a
+ * negative angle (commented-out reader, V63/main_recon.cu:199-224).  No
+ * context; fails with CT_ERR_INVALID_ARG if `size` bytes do not hold it. */
+ct_status ct_projection_path(const ct_geometry *g, int32_t view, int32_t tomo,
+                             const char *directory, char *buf, int32_t size);
+
+/* Reads views [view0, view0+nviews) from their files into the device stack
+ * proj[view-view0][row][col] of storage type dtype (F32/F16/BF16).  Each file
+ * holds float32 rows of ncols values; the first row0 rows are skipped (the
+ * reference's fseek, V63/main_recon.cu:219) and nrows rows are read.  A
+ * missing or short file fails with CT_ERR_IO naming the file. */
+ct_status ct_read_projections(ct_context *ctx, const ct_geometry *g, const char *directory,
+                              int32_t tomo, int32_t row0, void *proj, ct_dtype dtype,
+                              int32_t view0, int32_t nviews);
+
+/* Line integrals from raw detector counts, p = -ln(max((I - dark)/(flat - dark), floor))
+ * for nimg images of `pixels` pixels; flat and dark are one FP32 image of
+ * `pixels` each, flat NULL means the scalar i0 and dark NULL means 0.  A
+ * preprocessing step the reference leaves to the caller. */
+ct_status ct_log_transform(ct_context *ctx, const void *raw, ct_dtype raw_dtype,
+                           const float *flat, const float *dark, double i0,
+                           void *out, ct_dtype out_dtype, int64_t pixels, int32_t nimg,
+                           double floor_value);
+
+/* The volume as a raw float32 [z][y][x] file, the ".recon" format
+ * (V63/main_recon.cu:437-463; SaveDeviceDataToFile, V63/host_tool_functions.cu:2-15),
+ * from any storage type; and the reverse, e.g. to load an initial guess
+ * (V63/main_recon.cu:252-266). */
+ct_status ct_write_volume(ct_context *ctx, const ct_geometry *g, const void *vol,
+                          ct_dtype dtype, const char *path);
+ct_status ct_read_volume(ct_context *ctx, const ct_geometry *g, const char *path,
+                         void *vol, ct_dtype dtype);
+
+/* The reference's output names (V63/main_recon.cu:438,452,468): directory +
+ * "<folder>/<object>_<nx>_<ny>_<nz>_<vx>um_new_view_<nviews>_(<a0>,<a1>)_TV_.._L1_.._Lip_...recon"
+ * (iteration < 0) or "..._iterative_<k>_view_..." (iteration >= 0), and
+ * "<folder>/object_func_<object>_view_<nviews>_(<a0>,<a1>)_TV_.._Lip_...bin".
+ * unit_um converts the geometry's length unit to micrometres (1e6 for metres,
+ * 1e3 for millimetres); a1 = a0 + (nviews-1)*angle_step. */
+ct_status ct_recon_path(const ct_geometry *g, double unit_um, const char *directory,
+                        const char *output_folder, const char *object_name, int32_t iteration,
+                        double lambda_tv, double lambda_l1, double lipschitz, char *buf, int32_t size);
+ct_status ct_objective_path(const ct_geometry *g, const char *directory, const char *output_folder,
+                            const char *object_name, double lambda_tv, double lipschitz,
+                            char *buf, int32_t size);
+/* Writes `count` doubles, the (tv, data fidelity, objective) triples of the
+ * iterations so far, as the reference rewrites its object_func file each
+ * iteration (V63/main_recon.cu:467-477).  Host memory; no context. */
+ct_status ct_write_objective(const char *path, const double *values, int64_t count);
+
+/* FISTA state for exact resume, which the reference does not keep: x (the
+ * extrapolated point y_k), x_prev (x_{k-1}), t_state (t_{k-1}) and the
+ * iteration count, with the geometry and a checksum.  Continuing from a
+ * loaded checkpoint with ct_sart_fista_iteration gives the same bits as the
+ * uninterrupted run.  The file is written to path + ".part" and renamed into
+ * place.  Loading checks the format, storage type (CT_ERR_DTYPE), geometry
+ * (CT_ERR_GEOMETRY) and checksum (CT_ERR_IO). */
+ct_status ct_save_checkpoint(ct_context *ctx, const ct_geometry *g, const char *path,
+                             const void *x, const void *x_prev, ct_dtype dtype,
+                             double t_state, int32_t iteration);
+ct_status ct_load_checkpoint(ct_context *ctx, const ct_geometry *g, const char *path,
+                             void *x, void *x_prev, ct_dtype dtype,
+                             double *t_state, int32_t *iteration);
 
 /* ---- utilities -------------------------------------------------------- */
 /* Element conversion between storage types (n elements, device). */
