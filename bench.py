@@ -123,6 +123,72 @@ def cpu_baseline(g, meas_host, nviews, beta):
                       f"FP32, {dt:.1f} s, OpenMP {threads} threads"}
 
 
+def io_figures(ctx, g, meas32, x32) -> dict:
+    """The callers either side of the path, on the bench's own data: 64 view
+    files read back into HBM (from the page cache: the files were just
+    written), the log transform of the whole 1024-view stack of counts, and
+    the volume and FISTA-checkpoint files written and read."""
+    import tempfile
+
+    import torch
+    from gpu_amd import core
+
+    out = {}
+    nio = 64
+    gio = g.with_views(nio)
+    host = meas32[:nio].cpu().numpy()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    with tempfile.TemporaryDirectory() as td:
+        d = td + "/"
+        os.makedirs(d + "Projections")
+        for j in range(nio):
+            host[j].tofile(core.projection_path(gio, j, d))
+        buf = torch.empty((nio, g.nrows, g.ncols), device="cuda")
+        ctx.read_projections(gio, d, buf)                    # first pass warms the page cache
+        t0 = time.perf_counter()
+        ctx.read_projections(gio, d, buf)
+        t_read = time.perf_counter() - t0
+        assert torch.equal(buf, meas32[:nio])
+        out["read_views_GBps"] = round(buf.numel() * 4 / t_read / 1e9, 2)
+        del buf
+        counts = torch.exp(-meas32) * 1e5
+        lines = torch.empty_like(counts)
+        ctx.log_transform(counts, i0=1e5, out=lines)
+        torch.cuda.synchronize()
+        ev[0].record()
+        ctx.log_transform(counts, i0=1e5, out=lines)
+        ev[1].record()
+        torch.cuda.synchronize()
+        t_log = ev[0].elapsed_time(ev[1]) * 1e-3
+        gbps = counts.numel() * 8 / t_log / 1e9
+        out["log_transform"] = {"ms": round(t_log * 1e3, 3), "GBps": round(gbps, 1),
+                                "hbm_frac": round(gbps / HBM_PEAK_GBS, 3)}
+        del counts, lines
+        vol_path = d + "vol.recon"
+        t0 = time.perf_counter()
+        ctx.write_volume(g, x32, vol_path)
+        t_w = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        back = ctx.read_volume(g, vol_path)
+        t_r = time.perf_counter() - t0
+        assert torch.equal(back, x32)
+        out["volume_write_GBps"] = round(x32.numel() * 4 / t_w / 1e9, 2)
+        out["volume_read_GBps"] = round(x32.numel() * 4 / t_r / 1e9, 2)
+        ck = d + "state.ckpt"
+        t0 = time.perf_counter()
+        ctx.save_checkpoint(g, ck, x32, back, 1.5, 3)
+        t_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.load_checkpoint(g, ck, back, back.clone())
+        t_l = time.perf_counter() - t0
+        out["checkpoint_save_s"] = round(t_s, 3)
+        out["checkpoint_load_s"] = round(t_l, 3)
+    out["note"] = ("views: 64 raw float32 files of 1024^2 read through pinned buffers from the page cache; "
+                   "log transform: 1024 x 1024^2 counts -> line integrals, bytes = 8 per pixel; volume: 512^3 "
+                   "float32 .recon; checkpoint: x and x_prev with header and checksum")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +343,8 @@ def main() -> int:
                   "matched adjoint of the batch"}
     log(f"autograd pair: forward {pa['forward'][0] / nsamp:.3f} ms + adjoint {pa['adjoint'][0] / nsamp:.3f} ms per view")
     ctx.profile(False)
+    result["io"] = io_figures(ctx, g, meas32, x32)
+    log("files: " + ", ".join(f"{k} {v}" for k, v in result["io"].items() if k != "note"))
 
     if not args.no_fp16:
         log("FP16 path")

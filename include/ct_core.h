@@ -4,7 +4,9 @@
  * Every entry point takes DEVICE pointers (HBM-resident buffers owned by the
  * caller), runs asynchronously on the context's HIP stream unless it returns a
  * host scalar, and returns a ct_status.  No entry point exits the process or
- * stages host memory implicitly.  Host-array wrappers with the reference's own
+ * stages host memory implicitly; the file calls (projections in, volumes and
+ * checkpoints out) move data through pinned host buffers of their own and
+ * return when it is in place.  Host-array wrappers with the reference's own
  * function names are declared in ct_compat.h.
  *
  * Reference call surface replaced (paths relative to the reference root,
@@ -23,6 +25,7 @@
  *   tv_gradient_matrix_3d_kernel     V63/kernel_IterativeRecon_universal.cu:912
  *   Reconstruction_3D_ray_driven_TBCT TB/voxel/host_IterativeRecon_TBCT_voxel.c:1
  *   Forward_3D_ray_driven_siddon_TBCT TB/voxel/host_IterativeRecon_TBCT_voxel.c:111
+ *   projection files / outputs       V63/main_recon.cu:199-224 / :437-487
  */
 #ifndef CT_CORE_H
 #define CT_CORE_H
@@ -331,8 +334,9 @@ ct_status ct_read_projections(ct_context *ctx, const ct_geometry *g, const char 
 
 /* Line integrals from raw detector counts, p = -ln(max((I - dark)/(flat - dark), floor))
  * for nimg images of `pixels` pixels; flat and dark are one FP32 image of
- * `pixels` each, flat NULL means the scalar i0 and dark NULL means 0.  A
- * preprocessing step the reference leaves to the caller. */
+ * `pixels` each, flat NULL means the scalar i0 and dark NULL means 0; floor
+ * lies in [FLT_MIN, 1].  A preprocessing step the reference leaves to the
+ * caller. */
 ct_status ct_log_transform(ct_context *ctx, const void *raw, ct_dtype raw_dtype,
                            const float *flat, const float *dark, double i0,
                            void *out, ct_dtype out_dtype, int64_t pixels, int32_t nimg,
