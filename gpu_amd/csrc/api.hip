@@ -127,6 +127,8 @@ static ct_status validate(ct_context *ctx, const ct_geometry *g) {
         !std::isfinite(g->angle_shift_deg))
         return fail(ctx, CT_ERR_GEOMETRY, "geometry holds a non-finite value");
     if (!(g->dso > g->dod)) return fail(ctx, CT_ERR_GEOMETRY, "source must lie beyond the detector plane (dso > dod)");
+    if (g->style < CT_CONE || g->style > CT_PARALLEL) return fail(ctx, CT_ERR_GEOMETRY, "unknown beam style");
+    if (g->style != CT_CONE && g->nsources != 1) return fail(ctx, CT_ERR_GEOMETRY, "multi-source geometry is cone beam");
     // the source orbit must stay outside the volume's footprint
     const double bx = -g->vx * (g->nx / 2.0 + g->x_offset), ex = bx + g->vx * g->nx;
     const double hy = g->vy * g->ny / 2.0;
@@ -592,7 +594,7 @@ ct_status ct_fdk_backproject(ct_context *ctx, const ct_geometry *g, const void *
     CT_TRY(validate(ctx, g)); CT_TRY(dt_ok(ctx, pdt)); CT_TRY(dt_ok(ctx, vdt));
     CT_TRY(check_ptr(ctx, filt, pdt, "filtered")); CT_TRY(check_ptr(ctx, vol, vdt, "vol"));
     CT_TRY(views_ok(ctx, g, view0, nviews));
-    if (g->nsources != 1) return fail(ctx, CT_ERR_GEOMETRY, "FDK supports one source");
+    if (g->nsources != 1 || g->style != CT_CONE) return fail(ctx, CT_ERR_GEOMETRY, "FDK is single-source cone beam");
     DeviceGuard dg(ctx);
     // filtered holds the nviews images of views [view0, view0+nviews)
     return launch_fdk_bp(ctx, *g, filt, pdt, vol, vdt, view0, nviews);
@@ -603,7 +605,7 @@ ct_status ct_fdk(ct_context *ctx, const ct_geometry *g, const void *proj, ct_dty
     CT_CTX(ctx);
     CT_TRY(validate(ctx, g)); CT_TRY(dt_ok(ctx, pdt)); CT_TRY(dt_ok(ctx, vdt));
     CT_TRY(check_ptr(ctx, proj, pdt, "proj")); CT_TRY(check_ptr(ctx, vol, vdt, "vol"));
-    if (g->nsources != 1) return fail(ctx, CT_ERR_GEOMETRY, "FDK supports one source");
+    if (g->nsources != 1 || g->style != CT_CONE) return fail(ctx, CT_ERR_GEOMETRY, "FDK is single-source cone beam");
     if (window != CT_WINDOW_RAMP && window != CT_WINDOW_HANN) return fail(ctx, CT_ERR_INVALID_ARG, "unknown window");
     DeviceGuard dg(ctx);
     const ct_dtype fdt = vdt == CT_F64 ? CT_F64 : CT_F32;

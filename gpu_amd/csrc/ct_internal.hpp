@@ -128,16 +128,33 @@ inline bool dtype_valid(int d) { return d == CT_F32 || d == CT_F16 || d == CT_BF
 // Derived constants, computed on the host exactly as V63/main_recon.cu:61-111
 // computes its compile-time constants, in the arithmetic type T.
 template <typename T> struct DGeo {
-    int M, N, Z, R, ZP, NS, moff, tb;
+    int M, N, Z, R, ZP, NS, moff, tb, style;
     T vx, vy, vz, ivx, ivy, ivz, bx, by, bz;
     T du, dv, idu, idv, ymin, zmin, zmax, dso, dod, sx, sy, sz0, sdz;
     T us_rate, ang0, shift;
 };
 
+// Source end of the ray through the detector pixel at lateral coordinate u
+// and height v, per CT_style (V63/kernel_IterativeRecon_universal.cu:22-39):
+// cone beam one source (per source index for multi-source), fan beam at the
+// row's height, parallel beam moving with the pixel.
+template <typename T>
+__host__ __device__ __forceinline__ void ray_source(const DGeo<T> &d, T s, T c, T u, T v, int src, T &x, T &y,
+                                                    T &z) {
+    if (d.style == CT_PARALLEL) {
+        x = d.sx * c - u * s;
+        y = d.sx * s + u * c;
+    } else {
+        x = d.sx * c - d.sy * s;
+        y = d.sx * s + d.sy * c;
+    }
+    z = d.style == CT_CONE ? (d.tb ? (T)(d.sz0 + (T)src * d.sdz) : d.sz0) : v;
+}
+
 template <typename T> DGeo<T> derive(const ct_geometry &g) {
     DGeo<T> d;
     d.M = g.nx; d.N = g.ny; d.Z = g.nz; d.R = g.ncols; d.ZP = g.nrows;
-    d.NS = g.nsources < 1 ? 1 : g.nsources; d.moff = g.x_offset; d.tb = d.NS > 1;
+    d.NS = g.nsources < 1 ? 1 : g.nsources; d.moff = g.x_offset; d.tb = d.NS > 1; d.style = g.style;
     d.vx = (T)g.vx; d.vy = (T)g.vy; d.vz = (T)g.vz;
     d.ivx = (T)(1.0 / (double)d.vx); d.ivy = (T)(1.0 / (double)d.vy); d.ivz = (T)(1.0 / (double)d.vz);
     d.bx = -d.vx * ((T)d.M / (T)2 + (T)d.moff);

@@ -46,13 +46,12 @@ template <> __device__ __forceinline__ double loop_eps<double>() { return 1e-16;
 template <typename T, typename F>
 __device__ __forceinline__ T siddon_traverse(const DGeo<T> &d, T s, T c, int col, int row, int src, F &&visit) {
     const int M = d.M, N = d.N, Z = d.Z;
-    const T x2x = d.sx * c - d.sy * s;
-    const T x2y = d.sx * s + d.sy * c;
-    const T x2z = d.tb ? (T)(d.sz0 + (T)src * d.sdz) : d.sz0;
     const T u = d.ymin + (T)col * d.du;
     const T x1x = d.dod * c - u * s;
     const T x1y = d.dod * s + u * c;
     const T x1z = d.zmin + (T)row * d.dv;
+    T x2x, x2y, x2z;
+    ray_source(d, s, c, u, x1z, src, x2x, x2y, x2z);
     const T ixd = (T)1 / (x2x - x1x);
     const T iyd = (T)1 / (x2y - x1y);
     const T izd = (T)1 / (x2z - x1z);
@@ -139,7 +138,7 @@ __device__ __forceinline__ T siddon_traverse(const DGeo<T> &d, T s, T c, int col
 // Ray setup shared by the batched forward sum: the reference's entry point,
 // plane indices and first crossings (:20-292), kept operation for operation.
 template <typename T> struct RayState {
-    T amax, ac, ax, ay, az, dist, x1x, x1y, x1z, ixd, iyd, izd;
+    T amax, ac, ax, ay, az, dist, x1x, x1y, x1z, x2x, x2y, x2z, ixd, iyd, izd;
     int nx, ny, nz, vi, vj, vk, sgx, sgy, sgz;
     bool hit;
 };
@@ -148,13 +147,13 @@ template <typename T>
 __device__ __forceinline__ RayState<T> ray_setup(const DGeo<T> &d, T s, T c, int col, int row, int src) {
     RayState<T> r;
     const int M = d.M, N = d.N, Z = d.Z;
-    const T x2x = d.sx * c - d.sy * s;
-    const T x2y = d.sx * s + d.sy * c;
-    const T x2z = d.tb ? (T)(d.sz0 + (T)src * d.sdz) : d.sz0;
     const T u = d.ymin + (T)col * d.du;
     const T x1x = d.dod * c - u * s;
     const T x1y = d.dod * s + u * c;
     const T x1z = d.zmin + (T)row * d.dv;
+    T x2x, x2y, x2z;
+    ray_source(d, s, c, u, x1z, src, x2x, x2y, x2z);
+    r.x2x = x2x; r.x2y = x2y; r.x2z = x2z;
     const T ixd = (T)1 / (x2x - x1x);
     const T iyd = (T)1 / (x2y - x1y);
     const T izd = (T)1 / (x2z - x1z);
@@ -357,10 +356,9 @@ template <int ES, bool ZQ = false> struct FastWalk {
         RayState<float> r = ray_setup(d, s, c, col, row, src);
         if (!r.hit) return false;
         const unsigned M = d.M, N = d.N, Z = d.Z, MN = M * N;
-        const PlaneAxis X = plane_axis(d.vx, d.ivx, d.bx, r.x1x, r.ixd, r.sgx, d.vx * 1e-6f > fabsf(r.x1x - (d.sx * c - d.sy * s)));
-        const PlaneAxis Y = plane_axis(d.vy, d.ivy, d.by, r.x1y, r.iyd, r.sgy, d.vy * 1e-6f > fabsf(r.x1y - (d.sx * s + d.sy * c)));
-        const PlaneAxis Zx = plane_axis(d.vz, d.ivz, d.bz, r.x1z, r.izd, r.sgz,
-                                        d.vz * 1e-6f > fabsf(r.x1z - (d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0)));
+        const PlaneAxis X = plane_axis(d.vx, d.ivx, d.bx, r.x1x, r.ixd, r.sgx, d.vx * 1e-6f > fabsf(r.x1x - r.x2x));
+        const PlaneAxis Y = plane_axis(d.vy, d.ivy, d.by, r.x1y, r.iyd, r.sgy, d.vy * 1e-6f > fabsf(r.x1y - r.x2y));
+        const PlaneAxis Zx = plane_axis(d.vz, d.ivz, d.bz, r.x1z, r.izd, r.sgz, d.vz * 1e-6f > fabsf(r.x1z - r.x2z));
         // the walk runs over [alpha_min, alpha_max] of the reference setup,
         // clipped to the grid's entry and exit planes as this walk computes
         // them (so no step past the grid carries a nonzero length)
@@ -1022,6 +1020,7 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const flo
 // every voxel box ANC columns (the bound of bp_lds_fits without the
 // reconstruction cylinder: the adjoint covers every voxel).
 static bool adj_gather_fits(const DGeo<float> &d) {
+    if (d.style != CT_CONE) return false; // the column/row tables assume one source per view
     const double vx = d.vx, vy = d.vy, vz = d.vz, dso = d.dso, dod = d.dod;
     const double xs[2] = {(double)d.bx, (double)d.bx + d.M * vx}, ys[2] = {(double)d.by, (double)d.by + d.N * vy};
     double rho = 0;

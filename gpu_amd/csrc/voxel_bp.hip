@@ -60,9 +60,11 @@ __device__ __forceinline__ void voxel_wl(const DGeo<T> &d, T cx, T cy, T cz, T s
             const T t = px * c + py * s;
             const T sv = -px * s + py * c;
             const T ratio = (t - d.dod) / (d.dso - t);
-            const T u = ratio * (sv - d.sy) + sv;
-            const T w0 = ratio * (pz0 - szs) + pz0;
-            const T w1 = ratio * (pz1 - szs) + pz1;
+            // vertex on the detector per CT_style (:656-670): fan and parallel
+            // beam keep the voxel's height, parallel beam also its s
+            const T u = d.style == CT_PARALLEL ? sv : ratio * (sv - d.sy) + sv;
+            const T w0 = d.style == CT_CONE ? ratio * (pz0 - szs) + pz0 : cz;
+            const T w1 = d.style == CT_CONE ? ratio * (pz1 - szs) + pz1 : cz;
             minY = vmin(minY, u); maxY = vmax(maxY, u);
             minZ = vmin(minZ, vmin(w0, w1)); maxZ = vmax(maxZ, vmax(w0, w1));
         }
@@ -78,9 +80,10 @@ __device__ __forceinline__ void voxel_wl(const DGeo<T> &d, T cx, T cy, T cz, T s
     if (z0 <= 0) z0 = 0;
     if (z1 >= d.ZP - 1) z1 = d.ZP - 1;
 
+    // source per CT_style (:709-724); parallel beam: per column (:738-742)
     const T sxw = d.sx * c - d.sy * s;
     const T syw = d.sx * s + d.sy * c;
-    const T szw = szs;
+    const T szw = d.style == CT_CONE ? szs : cz;
     const T xl = cx - hvx, xh = cx + hvx, yl = cy - hvy, yh = cy + hvy, zl = cz - hvz, zh = cz + hvz;
     const T thr = d.vx * (T)1e-6;
     const int ncol = y1 - y0 + 1;
@@ -92,7 +95,9 @@ __device__ __forceinline__ void voxel_wl(const DGeo<T> &d, T cx, T cy, T cz, T s
                 const T u = d.ymin + (T)(y0 + cc) * d.du;
                 const T qx = d.dod * c - u * s;
                 const T qy = d.dod * s + u * c;
-                const T ex = sxw - qx, ey = syw - qy;
+                const T sxc = d.style == CT_PARALLEL ? d.sx * c - u * s : sxw;
+                const T syc = d.style == CT_PARALLEL ? d.sx * s + u * c : syw;
+                const T ex = sxc - qx, ey = syc - qy;
                 const T ax0 = (xl - qx) / ex, ax1 = (xh - qx) / ex;
                 const T ay0 = (yl - qy) / ey, ay1 = (yh - qy) / ey;
                 lo[cc] = vmax(vmin(ax0, ax1), vmin(ay0, ay1));
@@ -133,7 +138,9 @@ __device__ __forceinline__ void voxel_wl(const DGeo<T> &d, T cx, T cy, T cz, T s
                 const T u = d.ymin + (T)i * d.du;
                 const T qx = d.dod * c - u * s;
                 const T qy = d.dod * s + u * c;
-                const T ex = sxw - qx, ey = syw - qy;
+                const T sxc = d.style == CT_PARALLEL ? d.sx * c - u * s : sxw;
+                const T syc = d.style == CT_PARALLEL ? d.sx * s + u * c : syw;
+                const T ex = sxc - qx, ey = syc - qy;
                 const T ax0 = (xl - qx) / ex, ax1 = (xh - qx) / ex;
                 const T ay0 = (yl - qy) / ey, ay1 = (yh - qy) / ey;
                 const T amin = vmax(vmax(vmin(ax0, ax1), vmin(ay0, ay1)), loz);
@@ -178,9 +185,9 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
             const float t = px * c + py * s;
             const float sv = -px * s + py * c;
             const float ratio = (t - d.dod) * frcp(d.dso - t);
-            const float u = fmaf(ratio, sv - d.sy, sv);
-            const float w0 = fmaf(ratio, zl - szs, zl);
-            const float w1 = fmaf(ratio, zh - szs, zh);
+            const float u = d.style == CT_PARALLEL ? sv : fmaf(ratio, sv - d.sy, sv);
+            const float w0 = d.style == CT_CONE ? fmaf(ratio, zl - szs, zl) : cz;
+            const float w1 = d.style == CT_CONE ? fmaf(ratio, zh - szs, zh) : cz;
             minY = fminf(minY, u); maxY = fmaxf(maxY, u);
             minZ = fminf(minZ, fminf(w0, w1)); maxZ = fmaxf(maxZ, fmaxf(w0, w1));
         }
@@ -195,6 +202,7 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
     z0 = max(z0, 0); z1 = min(z1, d.ZP - 1);
     const float sxw = d.sx * c - d.sy * s;
     const float syw = d.sx * s + d.sy * c;
+    if (d.style != CT_CONE) szs = cz;
     const float thr = d.vx * 1e-6f;
     const int ncol = y1 - y0 + 1;
     if (ncol > MAXC) {
@@ -208,7 +216,8 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
             for (int i = y0; i <= y1; ++i) {
                 const float u = d.ymin + (float)i * d.du;
                 const float qx = d.dod * c - u * s, qy = d.dod * s + u * c;
-                const float ex = sxw - qx, ey = syw - qy;
+                const float ex = (d.style == CT_PARALLEL ? d.sx * c - u * s : sxw) - qx;
+                const float ey = (d.style == CT_PARALLEL ? d.sx * s + u * c : syw) - qy;
                 const float rx = frcp(ex), ry = frcp(ey);
                 const float ax0 = (xl - qx) * rx, ax1 = (xh - qx) * rx;
                 const float ay0 = (yl - qy) * ry, ay1 = (yh - qy) * ry;
@@ -229,7 +238,8 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
         if (cc < ncol) {
             const float u = d.ymin + (float)(y0 + cc) * d.du;
             const float qx = d.dod * c - u * s, qy = d.dod * s + u * c;
-            const float ex = sxw - qx, ey = syw - qy;
+            const float ex = (d.style == CT_PARALLEL ? d.sx * c - u * s : sxw) - qx;
+            const float ey = (d.style == CT_PARALLEL ? d.sx * s + u * c : syw) - qy;
             const float rx = frcp(ex), ry = frcp(ey);
             const float ax0 = (xl - qx) * rx, ax1 = (xh - qx) * rx;
             const float ay0 = (yl - qy) * ry, ay1 = (yh - qy) * ry;
@@ -786,6 +796,7 @@ static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *
 }
 
 static bool lds_eligible(const ct_context *ctx, const DGeo<float> &d, int nimg) {
+    if (d.style != CT_CONE) return false; // the footprint tables are the cone-beam projection
     return ctx->math == CT_MATH_FAST && nimg <= 65535 && bp_lds_fits(d, nimg == 1 ? zrun<true>() : zrun<false>());
 }
 

@@ -28,6 +28,7 @@ class CtGeometry(ctypes.Structure):
         ("angle_step_deg", ctypes.c_double), ("angle0_deg", ctypes.c_double),
         ("angle_shift_deg", ctypes.c_double),
         ("src_z0", ctypes.c_double), ("src_dz", ctypes.c_double),
+        ("style", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -54,6 +55,7 @@ class Geometry:
     angle_shift_deg: float = 0.0
     src_z0: float = 0.0
     src_dz: float = 0.0
+    style: int = 0                        # 0 cone beam, 1 multi-slice fan beam, 2 parallel beam
 
     def to_c(self) -> CtGeometry:
         step = self.angle_step_deg if self.angle_step_deg is not None else 360.0 / max(self.nviews, 1)
@@ -64,7 +66,7 @@ class Geometry:
             self.du, self.dv if self.dv is not None else self.du,
             self.det_offset, self.dso, self.dod,
             self.nviews, self.nsources, step, self.angle0_deg, self.angle_shift_deg,
-            self.src_z0, self.src_dz,
+            self.src_z0, self.src_dz, self.style, 0,
         )
 
     @property

@@ -98,7 +98,23 @@ typedef struct ct_geometry {
     double angle0_deg;            /* initialAngle */
     double angle_shift_deg;       /* shiftAngle */
     double src_z0, src_dz;        /* Source_z (CBCT) / Source_z_min, Source_interval (TB) */
+    int32_t style;                /* CT_style (V63/main_recon.cu:28): a ct_style value */
+    int32_t reserved;             /* 0 */
 } ct_geometry;
+
+/* Beam geometry (CT_style, V63/kernel_IterativeRecon_universal.cu:22-39,
+ * 656-670, 709-724).  CT_FAN: multi-slice fan beam, every detector row has
+ * its own source at the row's height, so rays run in z-planes.  CT_PARALLEL:
+ * the source moves with the detector pixel, at (dso*cos - u*sin,
+ * dso*sin + u*cos, v), so all rays of a view are parallel.  In the voxel
+ * backprojection both take the voxel's own height for the source and project
+ * without magnification along z (CT_PARALLEL also without it along u).
+ * Multi-source (nsources > 1) and FDK are cone-beam only. */
+typedef enum ct_style {
+    CT_CONE = 0,
+    CT_FAN = 1,
+    CT_PARALLEL = 2
+} ct_style;
 
 /* Ramp-filter window for FDK. */
 typedef enum ct_window {

@@ -39,7 +39,7 @@
 namespace {
 
 template <typename T> struct Geo {
-    int M, N, Z, R, ZP, NS, moff, nviews;
+    int M, N, Z, R, ZP, NS, moff, nviews, style;
     T vx, vy, vz, ivx, ivy, ivz, bx, by, bz;
     T du, dv, ymin, zmin, zmax, off, dso, dod, sx, sy, sz0, sdz;
     T us_rate, ang0, shift;
@@ -51,7 +51,7 @@ template <typename T> Geo<T> derive(const ct_geometry *g) {
     Geo<T> d;
     d.M = g->nx; d.N = g->ny; d.Z = g->nz; d.R = g->ncols; d.ZP = g->nrows;
     d.NS = g->nsources < 1 ? 1 : g->nsources; d.moff = g->x_offset; d.nviews = g->nviews;
-    d.tb = d.NS > 1;
+    d.tb = d.NS > 1; d.style = g->style;
     d.vx = (T)g->vx; d.vy = (T)g->vy; d.vz = (T)g->vz;
     d.ivx = (T)(1.0 / (double)d.vx); d.ivy = (T)(1.0 / (double)d.vy); d.ivz = (T)(1.0 / (double)d.vz);
     d.bx = -d.vx * ((T)d.M / (T)2 + (T)d.moff);
@@ -94,13 +94,20 @@ template <typename T, typename F>
 T siddon_ray(const Geo<T> &d, T s, T c, int col, int row, int src, F &&visit) {
     const int M = d.M, N = d.N, Z = d.Z;
     const T inf = std::numeric_limits<T>::infinity();
-    T x2x = d.sx * c - d.sy * s;
-    T x2y = d.sx * s + d.sy * c;
-    T x2z = source_z(d, src);
     T u = d.ymin + (T)col * d.du;
     T x1x = d.dod * c - u * s;
     T x1y = d.dod * s + u * c;
     T x1z = d.zmin + (T)row * d.dv;
+    // source per CT_style, V63/kernel_IterativeRecon_universal.cu:22-39
+    T x2x, x2y, x2z;
+    if (d.style == 2) {
+        x2x = d.sx * c - u * s;
+        x2y = d.sx * s + u * c;
+    } else {
+        x2x = d.sx * c - d.sy * s;
+        x2y = d.sx * s + d.sy * c;
+    }
+    x2z = d.style == 0 ? source_z(d, src) : x1z;
     T ixd = (T)1 / (x2x - x1x);
     T iyd = (T)1 / (x2y - x1y);
     T izd = (T)1 / (x2z - x1z);
@@ -282,8 +289,18 @@ void voxel_wl(const Geo<T> &d, int ix, int iy, int iz, T s, T c, int src, const 
                 T pz = cz + (T)k * d.vz - (T)0.5 * d.vz;
                 T t = px * c + py * s;
                 T sv = -px * s + py * c;
-                T ud = (t - d.dod) / (d.dso - t) * (sv - d.sy) + sv;
-                T wd = (t - d.dod) / (d.dso - t) * (pz - szs) + pz;
+                // vertex on the detector per CT_style, :656-670
+                T ud, wd;
+                if (d.style == 0) {
+                    ud = (t - d.dod) / (d.dso - t) * (sv - d.sy) + sv;
+                    wd = (t - d.dod) / (d.dso - t) * (pz - szs) + pz;
+                } else if (d.style == 1) {
+                    ud = (t - d.dod) / (d.dso - t) * (sv - d.sy) + sv;
+                    wd = cz;
+                } else {
+                    ud = sv;
+                    wd = cz;
+                }
                 minY = std::fmin(minY, ud); maxY = std::fmax(maxY, ud);
                 minZ = std::fmin(minZ, wd); maxZ = std::fmax(maxZ, wd);
             }
@@ -300,15 +317,20 @@ void voxel_wl(const Geo<T> &d, int ix, int iy, int iz, T s, T c, int src, const 
     if (y1 >= d.R - 1) y1 = d.R - 1;
     if (z0 <= 0) z0 = 0;
     if (z1 >= d.ZP - 1) z1 = d.ZP - 1;
+    // source per CT_style, :709-724 (CT_PARALLEL: per pixel, :738-742)
     T sx = d.sx * c - d.sy * s;
     T sy = d.sx * s + d.sy * c;
-    T sz = szs;
+    T sz = d.style == 0 ? szs : cz;
     for (int j = z0; j <= z1; j++)
         for (int i = y0; i <= y1; i++) {
             T u = d.ymin + (T)i * d.du;
             T qx = d.dod * c - u * s;
             T qy = d.dod * s + u * c;
             T qz = d.zmin + (T)j * d.dv;
+            if (d.style == 2) {
+                sx = d.sx * c - u * s;
+                sy = d.sx * s + u * c;
+            }
             T ax0 = ((cx - (T)0.5 * d.vx) - qx) / (sx - qx);
             T ax1 = ((cx + (T)0.5 * d.vx) - qx) / (sx - qx);
             T ay0 = ((cy - (T)0.5 * d.vy) - qy) / (sy - qy);

@@ -6,6 +6,7 @@ the matched adjoint passes the dot-product test to 1e-12 (FP64) / 1e-6 (FP32).
 """
 import ctypes
 import math
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -50,6 +51,10 @@ GEOMS = {
     # detector pixels much finer than the voxel footprint: the fast
     # backprojection leaves the LDS-staged kernel for the per-voxel one
     "fine": small_geom(du=0.15, dv=0.15, ncols=200, nrows=150, nviews=5),
+    # CT_style 1 and 2 (multi-slice fan beam, parallel beam), rows at the
+    # slice centres as the reference sets them up (Z_prj = ZETA, vz = du)
+    "fan": small_geom(style=1, nrows=16, dv=1.0),
+    "parallel": small_geom(style=2, nrows=16, dv=1.0, det_offset=0.7, x_offset=1),
 }
 
 
@@ -103,7 +108,7 @@ def test_fast_forward_accuracy_vs_fp64(ctx):
     assert e_gpu <= 1.25 * e_plain and e_gpu <= 2e-6
 
 
-@pytest.mark.parametrize("name", ["small", "aniso", "multisource"])
+@pytest.mark.parametrize("name", ["small", "aniso", "multisource", "fan", "parallel"])
 def test_forward_fp64(ctx, name):
     g = GEOMS[name]
     x = phantom(g, seed=12, scale=0.3).numpy().astype(np.float64)
@@ -123,7 +128,7 @@ def test_forward_plain_sart_geometry_view_subset(ctx):
         assert r <= 1e-5
 
 
-@pytest.mark.parametrize("name", ["small", "offsets", "multisource"])
+@pytest.mark.parametrize("name", ["small", "offsets", "multisource", "fan", "parallel"])
 def test_sart_correction(ctx, name):
     g = GEOMS[name]
     x = phantom(g, seed=13, scale=0.3).numpy()
@@ -134,7 +139,7 @@ def test_sart_correction(ctx, name):
 
 
 @pytest.mark.parametrize("math_mode", ["exact", "fast"])
-@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fine"])
+@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fine", "fan", "parallel"])
 @pytest.mark.parametrize("mode,nv", [(1, 1), (0, 1), (1, 3)])
 def test_backproject_voxel(ctx, name, mode, nv, math_mode):
     """exact: the reference's FP32 operation sequence, bit-identical to the
@@ -171,7 +176,8 @@ def test_backproject_voxel_fp64(ctx):
 
 
 @pytest.mark.parametrize("name,subset,order", [("small", 1, "ascending"), ("offsets", 3, "interleaved"),
-                                               ("multisource", 1, "ascending"), ("aniso", 4, "ascending")])
+                                               ("multisource", 1, "ascending"), ("aniso", 4, "ascending"),
+                                               ("fan", 1, "ascending"), ("parallel", 2, "interleaved")])
 def test_sart_sweep(ctx, name, subset, order):
     g = GEOMS[name]
     x = phantom(g, seed=15, scale=0.3).numpy()
@@ -205,7 +211,7 @@ def test_sart_plain_128_two_iterations(ctx):
     assert r <= 1e-4
 
 
-@pytest.mark.parametrize("name", ["small", "multisource"])
+@pytest.mark.parametrize("name", ["small", "multisource", "fan", "parallel"])
 def test_adjoint_matches_oracle(ctx, name):
     g = GEOMS[name]
     rng = np.random.default_rng(8)
@@ -227,7 +233,7 @@ def test_adjoint_dot_product_adjoint_64(ctx, dtype, tol):
     assert err <= tol
 
 
-@pytest.mark.parametrize("name", ["small", "offsets", "multisource", "fine"])
+@pytest.mark.parametrize("name", ["small", "offsets", "multisource", "fine", "fan", "parallel"])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-6)])
 def test_adjoint_dot_product_paths(ctx, name, dtype, tol):
     """Fast-mode FP32 adjoints: the gather form (every voxel sums the walk's
@@ -306,7 +312,7 @@ def test_norms(ctx):
         assert ctx.dot(A, B) == pytest.approx(float((host(A) * host(B)).sum()), rel=1e-11)
 
 
-@pytest.mark.parametrize("name", ["small", "multisource"])
+@pytest.mark.parametrize("name", ["small", "multisource", "parallel"])
 def test_objective(ctx, name):
     """Fused residual + TV norm equals the oracle's forward projection followed
     by the FP64 norms (V63/main_recon.cu:401-428)."""
@@ -402,6 +408,14 @@ def test_fdk_backprojection_paths(ctx, name):
     assert np.array_equal(exact, ref.astype(np.float64))
 
 
+def test_fdk_is_cone_beam_only(ctx):
+    for style in (1, 2):
+        g = replace(FDK_G, style=style)
+        with pytest.raises(core.CtError) as e:
+            ctx.fdk(g, torch.zeros(g.proj_shape, device="cuda"))
+        assert e.value.status == 2
+
+
 def test_ramp_filter_non_power_of_two(ctx):
     g = Geometry(nx=8, ny=8, nz=8, vx=1.0, ncols=45, nrows=7, du=0.9, nviews=3, dso=100.0, dod=-50.0)
     p = np.random.default_rng(2).standard_normal((3, 7, 45)).astype(np.float32)
@@ -475,7 +489,7 @@ def test_reduced_precision_storage(ctx, dt):
 
 
 @pytest.mark.parametrize("dt,tol", [(torch.float16, 3e-3), (torch.bfloat16, 1.2e-2)])
-@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource"])
+@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fan"])
 @pytest.mark.parametrize("mode,nv", [(0, 1), (1, 1), (1, 3)])
 def test_backproject_reduced_precision(ctx, dt, tol, name, mode, nv):
     """FP16/BF16 volumes take the packed-half chord engine: chord intervals in

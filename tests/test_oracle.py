@@ -27,11 +27,16 @@ def ray_endpoints(g, view, col, row, src=0):
     s, c = orc.view_angle(g, view, prec=64)
     vy = g.vy or g.vx
     sz = g.src_z0 + src * g.src_dz if g.nsources > 1 else g.src_z0
-    x2 = np.array([g.dso * c - g.det_offset * s, g.dso * s + g.det_offset * c, sz])
     u = -g.du * (g.ncols / 2 - 0.5) + g.det_offset + col * g.du
     dv = g.dv or g.du
     v = -dv * (g.nrows / 2 - 0.5) + row * dv
     x1 = np.array([g.dod * c - u * s, g.dod * s + u * c, v])
+    if g.style == 0:      # cone beam: one source
+        x2 = np.array([g.dso * c - g.det_offset * s, g.dso * s + g.det_offset * c, sz])
+    elif g.style == 1:    # multi-slice fan beam: the source sits at the row's height
+        x2 = np.array([g.dso * c - g.det_offset * s, g.dso * s + g.det_offset * c, v])
+    else:                 # parallel beam: the source moves with the pixel
+        x2 = np.array([g.dso * c - u * s, g.dso * s + u * c, v])
     return x1, x2
 
 
@@ -84,6 +89,9 @@ def oracle_segments(g, view, col, row, src=0):
     dict(x_offset=2, det_offset=1.3),
     dict(nx=9, ny=13, nz=5, vx=0.7, vy=1.1, vz=0.9, du=0.8, dv=1.2),
     dict(nsources=3, src_z0=-6.0, src_dz=6.0, nrows=8),
+    dict(style=1, nz=16, dv=1.0),
+    dict(style=2, nz=16, dv=1.0),
+    dict(style=2, x_offset=2, det_offset=1.3, nrows=11, dv=0.7),
 ])
 def test_siddon_matches_brute_force(geom_kw):
     g = small_geom(**geom_kw)
@@ -117,8 +125,9 @@ def chord_length(x1, x2, lo, hi):
     return max(0.0, a1 - a0) * np.linalg.norm(d)
 
 
-def test_uniform_volume_gives_chord_length():
-    g = small_geom()
+@pytest.mark.parametrize("style", [0, 1, 2])
+def test_uniform_volume_gives_chord_length(style):
+    g = small_geom(style=style)
     vol = np.ones(g.vol_shape)
     proj = orc.forward(g, vol, prec=64)
     lo = np.array([-g.nx / 2, -g.ny / 2, -g.nz / 2]) * g.vx
@@ -157,11 +166,14 @@ def test_voxel_backprojection_constant_correction():
     assert np.all(zero == 3.0)
 
 
-def test_voxel_chords_equal_siddon_lengths():
+@pytest.mark.parametrize("style", [0, 1, 2])
+def test_voxel_chords_equal_siddon_lengths(style):
     """The voxel-driven chord sum with c = 1 (W) equals A^T 1 built from the
     independent Siddon traversal, for every voxel inside the reconstruction
-    cylinder (the mask and the 1e-6 cut are the only differences)."""
-    g = small_geom(nx=10, ny=10, nz=6)
+    cylinder (the mask and the 1e-6 cut are the only differences).  Fan and
+    parallel beam project each voxel onto its own row (rows at the slice
+    centres, as the reference sets Z_prj = ZETA and vz = du)."""
+    g = small_geom(nx=10, ny=10, nz=6) if style == 0 else small_geom(nx=10, ny=10, nz=16, style=style)
     for view in (0, 3):
         W = orc.backproject_voxel(g, np.zeros(g.vol_shape), np.ones((1, g.nrows, g.ncols)), view0=view,
                                   nviews=1, beta=1.0, mode=0, prec=64)
