@@ -27,6 +27,7 @@ _DT = {torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16, torch.float
 ORDER = {"ascending": 0, "interleaved": 1}
 KERNELS = ("forward", "voxel_bp", "adjoint", "fgp", "fista", "fdk_bp", "ramp")
 WINDOW = {"ramp": 0, "hann": 1}
+FDK_SOURCES = {"inverse_square": 0, "nearest": 1, "equal": 2}
 
 
 class CtError(RuntimeError):
@@ -73,6 +74,8 @@ def _load() -> ctypes.CDLL:
         "ct_ramp_filter": ([P, G, P, P, I, I, I], I),
         "ct_fdk_backproject": ([P, G, P, I, P, I, I, I], I),
         "ct_fdk": ([P, G, P, I, P, I, I], I),
+        "ct_fdk_ms": ([P, G, P, I, P, I, I, I], I),
+        "ct_fdk_backproject_ms": ([P, G, P, I, P, I, I, I, I], I),
         "ct_sart_fista_iteration": ([P, G, P, P, I, P, I, D, I, I, D, I, DP], I),
         "ct_convert": ([P, P, I, P, I, I64], I),
         "ct_objective": ([P, G, P, I, P, I, D, DP, DP, DP], I),
@@ -352,21 +355,24 @@ class Context:
         return out
 
     def fdk_backproject(self, g: Geometry, filt: torch.Tensor, vol: torch.Tensor, view0: int = 0,
-                        nviews: int | None = None) -> torch.Tensor:
+                        nviews: int | None = None, sources: str = "inverse_square") -> torch.Tensor:
+        """Multi-source geometries combine their sources per view as `sources` says
+        (TB/FDK: inverse_square = kernel_FDK.cu, nearest = OneRay, equal = EqualWeighting)."""
         nv = g.nviews - view0 if nviews is None else nviews
         self._stream()
         gc = g.to_c()
-        self._check(_lib.ct_fdk_backproject(self._h, ctypes.byref(gc), _ptr(filt), dtype_code(filt), _ptr(vol),
-                                            dtype_code(vol), view0, nv))
+        self._check(_lib.ct_fdk_backproject_ms(self._h, ctypes.byref(gc), _ptr(filt), dtype_code(filt), _ptr(vol),
+                                               dtype_code(vol), view0, nv, FDK_SOURCES[sources]))
         return vol
 
-    def fdk(self, g: Geometry, proj: torch.Tensor, vol: torch.Tensor | None = None, window: str = "ramp") -> torch.Tensor:
+    def fdk(self, g: Geometry, proj: torch.Tensor, vol: torch.Tensor | None = None, window: str = "ramp",
+            sources: str = "inverse_square") -> torch.Tensor:
         if vol is None:
             vol = torch.zeros(g.vol_shape, device=proj.device, dtype=proj.dtype)
         self._stream()
         gc = g.to_c()
-        self._check(_lib.ct_fdk(self._h, ctypes.byref(gc), _ptr(proj), dtype_code(proj), _ptr(vol), dtype_code(vol),
-                                WINDOW[window]))
+        self._check(_lib.ct_fdk_ms(self._h, ctypes.byref(gc), _ptr(proj), dtype_code(proj), _ptr(vol),
+                                   dtype_code(vol), WINDOW[window], FDK_SOURCES[sources]))
         return vol
 
     # ---- driver

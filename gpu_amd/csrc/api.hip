@@ -588,33 +588,46 @@ ct_status ct_ramp_filter(ct_context *ctx, const ct_geometry *g, const void *in, 
     return launch_ramp(ctx, *g, in, dt, out, dt, nimg, window);
 }
 
-ct_status ct_fdk_backproject(ct_context *ctx, const ct_geometry *g, const void *filt, ct_dtype pdt, void *vol,
-                             ct_dtype vdt, int32_t view0, int32_t nviews) {
+ct_status ct_fdk_backproject_ms(ct_context *ctx, const ct_geometry *g, const void *filt, ct_dtype pdt, void *vol,
+                                ct_dtype vdt, int32_t view0, int32_t nviews, ct_fdk_sources sources) {
     CT_CTX(ctx);
     CT_TRY(validate(ctx, g)); CT_TRY(dt_ok(ctx, pdt)); CT_TRY(dt_ok(ctx, vdt));
     CT_TRY(check_ptr(ctx, filt, pdt, "filtered")); CT_TRY(check_ptr(ctx, vol, vdt, "vol"));
     CT_TRY(views_ok(ctx, g, view0, nviews));
-    if (g->nsources != 1 || g->style != CT_CONE) return fail(ctx, CT_ERR_GEOMETRY, "FDK is single-source cone beam");
+    if (g->style != CT_CONE) return fail(ctx, CT_ERR_GEOMETRY, "FDK is cone beam");
+    if (sources < CT_FDK_INVERSE_SQUARE || sources > CT_FDK_EQUAL) return fail(ctx, CT_ERR_INVALID_ARG, "unknown source combination");
     DeviceGuard dg(ctx);
-    // filtered holds the nviews images of views [view0, view0+nviews)
-    return launch_fdk_bp(ctx, *g, filt, pdt, vol, vdt, view0, nviews);
+    // filtered holds the images of views [view0, view0+nviews), all sources of each
+    return launch_fdk_bp(ctx, *g, filt, pdt, vol, vdt, view0, nviews, (int)sources);
 }
 
-ct_status ct_fdk(ct_context *ctx, const ct_geometry *g, const void *proj, ct_dtype pdt, void *vol, ct_dtype vdt,
-                 ct_window window) {
+ct_status ct_fdk_backproject(ct_context *ctx, const ct_geometry *g, const void *filt, ct_dtype pdt, void *vol,
+                             ct_dtype vdt, int32_t view0, int32_t nviews) {
+    return ct_fdk_backproject_ms(ctx, g, filt, pdt, vol, vdt, view0, nviews, CT_FDK_INVERSE_SQUARE);
+}
+
+ct_status ct_fdk_ms(ct_context *ctx, const ct_geometry *g, const void *proj, ct_dtype pdt, void *vol, ct_dtype vdt,
+                    ct_window window, ct_fdk_sources sources) {
     CT_CTX(ctx);
     CT_TRY(validate(ctx, g)); CT_TRY(dt_ok(ctx, pdt)); CT_TRY(dt_ok(ctx, vdt));
     CT_TRY(check_ptr(ctx, proj, pdt, "proj")); CT_TRY(check_ptr(ctx, vol, vdt, "vol"));
-    if (g->nsources != 1 || g->style != CT_CONE) return fail(ctx, CT_ERR_GEOMETRY, "FDK is single-source cone beam");
+    if (g->style != CT_CONE) return fail(ctx, CT_ERR_GEOMETRY, "FDK is cone beam");
     if (window != CT_WINDOW_RAMP && window != CT_WINDOW_HANN) return fail(ctx, CT_ERR_INVALID_ARG, "unknown window");
+    if (sources < CT_FDK_INVERSE_SQUARE || sources > CT_FDK_EQUAL) return fail(ctx, CT_ERR_INVALID_ARG, "unknown source combination");
     DeviceGuard dg(ctx);
     const ct_dtype fdt = vdt == CT_F64 ? CT_F64 : CT_F32;
     const int64_t P = pix(g) * g->nviews;
     ct_status st;
     void *w = slot(ctx, SLOT_FDK, dtype_size(fdt) * P, &st);
     if (!w) return st;
-    CT_TRY(launch_ramp(ctx, *g, proj, pdt, w, fdt, g->nviews, window, true)); // cosine weight + filter
-    return launch_fdk_bp(ctx, *g, w, fdt, vol, vdt, 0, g->nviews);
+    const int nimg = g->nviews * (g->nsources < 1 ? 1 : g->nsources);
+    CT_TRY(launch_ramp(ctx, *g, proj, pdt, w, fdt, nimg, window, true)); // cosine weight + filter
+    return launch_fdk_bp(ctx, *g, w, fdt, vol, vdt, 0, g->nviews, (int)sources);
+}
+
+ct_status ct_fdk(ct_context *ctx, const ct_geometry *g, const void *proj, ct_dtype pdt, void *vol, ct_dtype vdt,
+                 ct_window window) {
+    return ct_fdk_ms(ctx, g, proj, pdt, vol, vdt, window, CT_FDK_INVERSE_SQUARE);
 }
 
 ct_status ct_sart_fista_iteration(ct_context *ctx, const ct_geometry *g, void *x, void *x_prev, ct_dtype vdt,

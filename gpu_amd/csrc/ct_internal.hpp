@@ -305,7 +305,8 @@ ct_status launch_fdk_weight(ct_context *ctx, const ct_geometry &g, const void *i
 // weighted: apply the FDK cosine weight to the input rows first
 ct_status launch_ramp(ct_context *ctx, const ct_geometry &g, const void *in, ct_dtype idt, void *out, ct_dtype odt,
                       int32_t nimg, ct_window window, bool weighted = false);
+// sources: how a multi-source geometry combines its sources (ct_fdk_sources)
 ct_status launch_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, ct_dtype pdt, void *vol, ct_dtype vdt,
-                        int32_t view0, int32_t nviews);
+                        int32_t view0, int32_t nv, int sources = 0);
 
 } // namespace ct

@@ -21,7 +21,13 @@ template <typename S> __device__ __forceinline__ float ldx(const S *p, int64_t i
     else return ldf(p, i);
 }
 
-// cosine weight, kernel_FDK_projection_adjust (V63/FDK/kernel_FDK.cu:1-14)
+// cosine weight, kernel_FDK_projection_adjust (V63/FDK/kernel_FDK.cu:1-14);
+// multi-source images [view][source] measure the row height from their
+// source (TB/FDK/kernel_FDK.cu:1-17)
+template <typename T> __device__ __forceinline__ T fdk_row_height(const DGeo<T> &d, int z, int img) {
+    return d.tb ? d.zmin + (T)z * d.dv - (T)(d.sz0 + (T)(img % d.NS) * d.sdz) : d.zmin + (T)z * d.dv;
+}
+
 template <typename T, typename SI, typename SO>
 __global__ void k_fdk_weight(DGeo<T> d, const SI *in, SO *out, int nviews) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,7 +35,7 @@ __global__ void k_fdk_weight(DGeo<T> d, const SI *in, SO *out, int nviews) {
     if (x >= d.R) return;
     const T x1 = d.dso;
     const T x2 = (d.ymin + (T)x * d.du) * d.dso / (d.dso - d.dod);
-    const T x3 = (d.zmin + (T)z * d.dv) * d.dso / (d.dso - d.dod);
+    const T x3 = fdk_row_height(d, z, v) * d.dso / (d.dso - d.dod);
     const int64_t idx = ((int64_t)v * d.ZP + z) * d.R + x;
     T a;
     if constexpr (std::is_same<SI, double>::value) a = (T)in[idx]; else a = (T)ldf(in, idx);
@@ -238,10 +244,10 @@ template <int LOGN> __device__ __forceinline__ constexpr int sh_perm(int i) {
 }
 
 template <typename SI>
-__device__ __forceinline__ float fdk_weighted(const DGeo<float> &d, const SI *in, int64_t idx, int x, int z) {
+__device__ __forceinline__ float fdk_weighted(const DGeo<float> &d, const SI *in, int64_t idx, int x, int z, int img) {
     const float x1 = d.dso;
     const float x2 = (d.ymin + (float)x * d.du) * d.dso / (d.dso - d.dod);
-    const float x3 = (d.zmin + (float)z * d.dv) * d.dso / (d.dso - d.dod);
+    const float x3 = fdk_row_height(d, z, img) * d.dso / (d.dso - d.dod);
     const float a = ldx(in, idx);
     return a * d.dso / sqrtf(x1 * x1 + x2 * x2 + x3 * x3);
 }
@@ -259,6 +265,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_ramp_sh(const SI *__restric
     const int64_t ra = (int64_t)blockIdx.x * 2, rb = ra + 1;
     const bool hb = rb < nrows_total;
     const int za = (int)(ra % d.ZP), zb = (int)(rb % d.ZP);
+    const int ia = (int)(ra / d.ZP), ib = (int)(rb / d.ZP); // image (view, source) of each row
     float2 v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -266,8 +273,8 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_ramp_sh(const SI *__restric
         float a = 0.0f, b = 0.0f;
         if (n < R) {
             if constexpr (WEIGHT) {
-                a = fdk_weighted(d, in, ra * R + n, n, za);
-                if (hb) b = fdk_weighted(d, in, rb * R + n, n, zb);
+                a = fdk_weighted(d, in, ra * R + n, n, za, ia);
+                if (hb) b = fdk_weighted(d, in, rb * R + n, n, zb, ib);
             } else {
                 a = ldx(in, ra * R + n);
                 if (hb) b = ldx(in, rb * R + n);
@@ -463,7 +470,7 @@ ct_status launch_ramp(ct_context *ctx, const ct_geometry &g, const void *in, ct_
 
 template <typename SI, typename SO>
 static ct_status run_weight(ct_context *ctx, const ct_geometry &g, const void *in, void *out) {
-    dim3 b(256), gr((g.ncols + 255) / 256, g.nrows, g.nviews);
+    dim3 b(256), gr((g.ncols + 255) / 256, g.nrows, g.nviews * (g.nsources < 1 ? 1 : g.nsources));
     if (std::is_same<SI, double>::value && std::is_same<SO, double>::value)
         hipLaunchKernelGGL((k_fdk_weight<double, SI, SO>), gr, b, 0, ctx->stream, derive<double>(g), (const SI *)in, (SO *)out, g.nviews);
     else
@@ -497,7 +504,7 @@ ct_status launch_fdk_weight(ct_context *ctx, const ct_geometry &g, const void *i
 // with the FP32 factor dtheta*360/(us_rate*Nviews).
 template <typename T, typename SP, typename SV>
 __global__ __launch_bounds__(256) void k_fdk_bp(DGeo<T> d, const T *__restrict__ ang, const SP *__restrict__ filt,
-                                                SV *__restrict__ vol, int view0, int nv, T wscale) {
+                                                SV *__restrict__ vol, int view0, int nv, T dth, T usn) {
     const int ix = blockIdx.x * 64 + threadIdx.x;
     const int iy = blockIdx.y * 4 + threadIdx.y;
     const int iz = blockIdx.z;
@@ -533,7 +540,90 @@ __global__ __launch_bounds__(256) void k_fdk_bp(DGeo<T> d, const T *__restrict__
             t2 = ((T)1 - an0) * ((T)1 - bn0) * p00 + ((T)1 - an0) * bn0 * p01 + an0 * ((T)1 - bn0) * p10 +
                  an0 * bn0 * p11;
         }
-        acc += t1 * t2 * wscale;
+        // temp1*temp2 * (us_rate*PI/180) * 360/(us_rate*Nviews), V63/FDK/kernel_FDK.cu:60
+        acc = acc + t1 * t2 * dth * (T)360 / usn;
+    }
+    if constexpr (std::is_same<SV, double>::value) vol[idx] = acc; else St<SV>::st(vol, idx, (float)acc);
+}
+
+// Multi-source FDK backprojection (TB/FDK/*/kernel_FDK.cu), one voxel per
+// lane, the reference's operation order in both arithmetic modes: per view
+// the sources combine as `mode` says - 0: sources whose stencil is on the
+// detector weighted by 1/(z - z_s)^2 and normalised (TB/FDK/kernel_FDK.cu:37-99);
+// 1: the source nearest in z among those whose row is on the detector
+// (TB/FDK/OneRay/kernel_FDK.cu:37-113); 2: their sum
+// (TB/FDK/EqualWeighting/kernel_FDK.cu:37-91, without that kernel's
+// unsynchronised per-source updates) - and the view's value is added to the
+// volume (kernel_volume_divide, TB/FDK/kernel_FDK.cu:21-33).
+template <typename T, typename SP, typename SV>
+__global__ __launch_bounds__(256) void k_fdk_bp_ms(DGeo<T> d, const T *__restrict__ ang, const SP *__restrict__ filt,
+                                                   SV *__restrict__ vol, int view0, int nv, T dth, T usn, int mode) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y * 4 + threadIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= d.M || iy >= d.N) return;
+    const T x = d.bx + d.vx * (T)0.5 + (T)ix * d.vx;
+    const T y = d.by + d.vy * (T)0.5 + (T)iy * d.vy;
+    const T z = d.bz + d.vz * (T)0.5 + (T)iz * d.vz;
+    const int64_t P = (int64_t)d.R * d.ZP;
+    const int64_t idx = ((int64_t)iz * d.N + iy) * d.M + ix;
+    T acc;
+    if constexpr (std::is_same<SV, double>::value) acc = vol[idx]; else acc = (T)ldf(vol, idx);
+    for (int q = 0; q < nv; ++q) {
+        const int v = view0 + q;
+        const T s = ang[2 * v], c = ang[2 * v + 1];
+        const T t = x * c + y * s;
+        const T sv = -x * s + y * c;
+        const T t1 = d.dso * d.dso / ((d.dso - sv) * (d.dso - sv));
+        // ksi of source i (the stencil row), p (column)
+        auto ksi_of = [&](T sz) {
+            const T k = d.dso * (z - sz) / (d.dso - sv);
+            return (k * (d.dso - d.dod) / d.dso + sz - d.zmin) / d.dv;
+        };
+        T p = d.dso * t / (d.dso - sv);
+        p = (p * (d.dso - d.dod) / d.dso - d.ymin) / d.du;
+        const bool pin = ((T)0 <= p) && (p + (T)1 < (T)d.R);
+        auto sample = [&](int i, T ksi) {
+            const int an = (int)floor(ksi), bn = (int)floor(p);
+            const T an0 = ksi - (T)an, bn0 = p - (T)bn;
+            const SP *pr = filt + ((int64_t)q * d.NS + i) * P + (int64_t)an * d.R + bn;
+            T p00, p01, p10, p11;
+            if constexpr (std::is_same<SP, double>::value) { p00 = pr[0]; p01 = pr[1]; p10 = pr[d.R]; p11 = pr[d.R + 1]; }
+            else { p00 = ldf(pr, 0); p01 = ldf(pr, 1); p10 = ldf(pr, d.R); p11 = ldf(pr, d.R + 1); }
+            return ((T)1 - an0) * ((T)1 - bn0) * p00 + ((T)1 - an0) * bn0 * p01 + an0 * ((T)1 - bn0) * p10 +
+                   an0 * bn0 * p11;
+        };
+        T upd = 0;
+        if (mode == 1) {
+            int sel = -1;
+            T zsel = 0, ksel = 0;
+            for (int i = 0; i < d.NS; ++i) {
+                const T sz = (T)(d.sz0 + (T)i * d.sdz);
+                const T ksi = ksi_of(sz);
+                if (((T)0 <= ksi) && (ksi + (T)1 < (T)d.ZP) && (sel < 0 || fabs(sz - z) < fabs(zsel - z))) {
+                    sel = i; zsel = sz; ksel = ksi;
+                }
+            }
+            if (sel >= 0 && pin) upd = t1 * sample(sel, ksel) * dth * (T)360 / usn;
+        } else if (pin) {
+            T wsum = 0;
+            for (int i = 0; i < d.NS; ++i) {
+                const T sz = (T)(d.sz0 + (T)i * d.sdz);
+                const T ksi = ksi_of(sz);
+                if (!(((T)0 <= ksi) && (ksi + (T)1 < (T)d.ZP))) continue;
+                const T term = t1 * sample(i, ksi) * dth * (T)360 / usn;
+                if (mode == 0) {
+                    const T dz = fabs(z - sz);
+                    const T w = (T)1 / dz / dz;
+                    wsum += w;
+                    upd += term * w;
+                } else {
+                    upd += term;
+                }
+            }
+            if (mode == 0 && wsum > (T)0) upd /= wsum;
+        }
+        acc = acc + upd;
     }
     if constexpr (std::is_same<SV, double>::value) vol[idx] = acc; else St<SV>::st(vol, idx, (float)acc);
 }
@@ -808,16 +898,23 @@ static bool fdk_lds_fits(const DGeo<float> &d) {
 }
 
 template <typename T, typename SP, typename SV>
-static ct_status run_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, void *vol, int view0, int nv) {
+static ct_status run_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, void *vol, int view0, int nv,
+                            int sources) {
     ct_status st;
     const T *ang = (const T *)angle_table(ctx, g, sizeof(T) == 8, true, &st);
     if (!ang) return st;
     DGeo<T> d = derive<T>(g);
     const T pi = sizeof(T) == 8 ? (T)3.141592653589793 : (T)3.141592653589793f;
     const T dth = d.us_rate * pi / (T)180;
-    const double w = (double)dth * 360.0 / (double)(d.us_rate * (T)g.nviews);
+    const T usn = d.us_rate * (T)g.nviews;
+    const double w = (double)dth * 360.0 / (double)usn;
     dim3 block(64, 4, 1), grid((d.M + 63) / 64, (d.N + 3) / 4, d.Z);
     ProfScope ps(ctx, CT_K_FDK_BP);
+    if (d.NS > 1) {
+        hipLaunchKernelGGL((k_fdk_bp_ms<T, SP, SV>), grid, block, 0, ctx->stream, d, ang, (const SP *)filt, (SV *)vol,
+                           view0, nv, dth, usn, sources);
+        return launch_check(ctx, "fdk backprojection");
+    }
     if constexpr (std::is_same<T, float>::value && !std::is_same<SP, double>::value) {
         if (ctx->math == CT_MATH_FAST && (uint64_t)g.ncols * g.nrows * sizeof(SP) < (1ull << 32)) {
             float4 *vt = (float4 *)slot(ctx, SLOT_FDKV, sizeof(float4) * (size_t)nv, &st);
@@ -837,32 +934,33 @@ static ct_status run_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *f
         }
     }
     hipLaunchKernelGGL((k_fdk_bp<T, SP, SV>), grid, block, 0, ctx->stream, d, ang, (const SP *)filt, (SV *)vol, view0,
-                       nv, (T)w);
+                       nv, dth, usn);
     return launch_check(ctx, "fdk backprojection");
 }
 
 template <typename SP>
-static ct_status fdk_bp_vol(ct_context *ctx, const ct_geometry &g, const void *filt, void *vol, ct_dtype vdt, int v0, int nv) {
+static ct_status fdk_bp_vol(ct_context *ctx, const ct_geometry &g, const void *filt, void *vol, ct_dtype vdt, int v0,
+                            int nv, int sources) {
     switch (vdt) {
-    case CT_F32: return run_fdk_bp<float, SP, float>(ctx, g, filt, vol, v0, nv);
-    case CT_F16: return run_fdk_bp<float, SP, __half>(ctx, g, filt, vol, v0, nv);
-    case CT_BF16: return run_fdk_bp<float, SP, __hip_bfloat16>(ctx, g, filt, vol, v0, nv);
+    case CT_F32: return run_fdk_bp<float, SP, float>(ctx, g, filt, vol, v0, nv, sources);
+    case CT_F16: return run_fdk_bp<float, SP, __half>(ctx, g, filt, vol, v0, nv, sources);
+    case CT_BF16: return run_fdk_bp<float, SP, __hip_bfloat16>(ctx, g, filt, vol, v0, nv, sources);
     case CT_F64: break;
     }
     return fail(ctx, CT_ERR_DTYPE, "fdk backprojection: F64 volumes need F64 projections");
 }
 
 ct_status launch_fdk_bp(ct_context *ctx, const ct_geometry &g, const void *filt, ct_dtype pdt, void *vol, ct_dtype vdt,
-                        int32_t view0, int32_t nv) {
+                        int32_t view0, int32_t nv, int sources) {
     if (nv <= 0) return CT_OK;
     if (pdt == CT_F64 || vdt == CT_F64) {
         if (pdt != CT_F64 || vdt != CT_F64) return fail(ctx, CT_ERR_DTYPE, "fdk backprojection: mixed F64 dtypes");
-        return run_fdk_bp<double, double, double>(ctx, g, filt, vol, view0, nv);
+        return run_fdk_bp<double, double, double>(ctx, g, filt, vol, view0, nv, sources);
     }
     switch (pdt) {
-    case CT_F32: return fdk_bp_vol<float>(ctx, g, filt, vol, vdt, view0, nv);
-    case CT_F16: return fdk_bp_vol<__half>(ctx, g, filt, vol, vdt, view0, nv);
-    case CT_BF16: return fdk_bp_vol<__hip_bfloat16>(ctx, g, filt, vol, vdt, view0, nv);
+    case CT_F32: return fdk_bp_vol<float>(ctx, g, filt, vol, vdt, view0, nv, sources);
+    case CT_F16: return fdk_bp_vol<__half>(ctx, g, filt, vol, vdt, view0, nv, sources);
+    case CT_BF16: return fdk_bp_vol<__hip_bfloat16>(ctx, g, filt, vol, vdt, view0, nv, sources);
     default: break;
     }
     return fail(ctx, CT_ERR_DTYPE, "fdk backprojection: bad projection dtype");

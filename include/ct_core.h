@@ -269,16 +269,34 @@ ct_status ct_ramp_filter(ct_context *ctx, const ct_geometry *g,
                          const void *in, void *out, ct_dtype dtype,
                          int32_t nimg, ct_window window);
 /* Weighted bilinear backprojection of filtered views [view0, view0+nviews)
- * accumulated into vol (kernel_FDK_reconstruction, V63/FDK/kernel_FDK.cu:16-62). */
+ * accumulated into vol (kernel_FDK_reconstruction, V63/FDK/kernel_FDK.cu:16-62).
+ * A multi-source geometry takes the CT_FDK_INVERSE_SQUARE combination. */
 ct_status ct_fdk_backproject(ct_context *ctx, const ct_geometry *g,
                              const void *filtered, ct_dtype proj_dtype,
                              void *vol, ct_dtype vol_dtype,
                              int32_t view0, int32_t nviews);
 /* Whole FDK (Reconstruction_FDK, V63/host_FDK.cpp:4-57): vol += FDK(proj).
- * Scratch for the filtered stack is held by the context. */
+ * Scratch for the filtered stack is held by the context.  Cone beam only. */
 ct_status ct_fdk(ct_context *ctx, const ct_geometry *g,
                  const void *proj, ct_dtype proj_dtype,
                  void *vol, ct_dtype vol_dtype, ct_window window);
+
+/* Multi-source FDK (TB/FDK/host_FDK.cpp:4-83): the stack holds
+ * [view][source][row][col]; each image is weighted (rows measured from its
+ * source's height) and filtered, and per view and voxel the sources combine
+ * as the reference's three variants do. */
+typedef enum ct_fdk_sources {
+    CT_FDK_INVERSE_SQUARE = 0,  /* sources on the detector weighted by 1/(z - z_s)^2, normalised (TB/FDK/kernel_FDK.cu:37-99) */
+    CT_FDK_NEAREST = 1,         /* the source nearest in z whose row is on the detector (TB/FDK/OneRay/kernel_FDK.cu:37-113) */
+    CT_FDK_EQUAL = 2            /* the sum over sources (TB/FDK/EqualWeighting/kernel_FDK.cu:37-91) */
+} ct_fdk_sources;
+ct_status ct_fdk_backproject_ms(ct_context *ctx, const ct_geometry *g,
+                                const void *filtered, ct_dtype proj_dtype,
+                                void *vol, ct_dtype vol_dtype,
+                                int32_t view0, int32_t nviews, ct_fdk_sources sources);
+ct_status ct_fdk_ms(ct_context *ctx, const ct_geometry *g,
+                    const void *proj, ct_dtype proj_dtype,
+                    void *vol, ct_dtype vol_dtype, ct_window window, ct_fdk_sources sources);
 
 /* ---- driver ----------------------------------------------------------- */
 /* One outer iteration of SART + TV-FGP + FISTA momentum (V63/main_recon.cu:344-385),

@@ -664,15 +664,19 @@ template <typename T> void tv_eps_grad_impl(int M, int N, int Z, const T *f, dou
 }
 
 /* ---- FDK (V63/host_FDK.cpp, V63/FDK/kernel_FDK.cu, V63/FDK/FFT_1d_GPU.h) ---- */
+/* Cosine weight, V63/FDK/kernel_FDK.cu:1-14; multi-source images
+ * [view][source] measure rows from the source's height,
+ * TB/FDK/kernel_FDK.cu:1-17. */
 template <typename T> void fdk_weight_impl(const Geo<T> &d, const T *in, T *out) {
     const int64_t P = (int64_t)d.R * d.ZP;
 #pragma omp parallel for collapse(2)
-    for (int v = 0; v < d.nviews; ++v)
+    for (int v = 0; v < d.nviews * d.NS; ++v)
         for (int z = 0; z < d.ZP; ++z)
             for (int x = 0; x < d.R; ++x) {
                 T x1 = d.dso;
                 T x2 = (d.ymin + (T)x * d.du) * d.dso / (d.dso - d.dod);
-                T x3 = (d.zmin + (T)z * d.dv) * d.dso / (d.dso - d.dod);
+                T x3 = d.tb ? (d.zmin + (T)z * d.dv - source_z(d, v % d.NS)) * d.dso / (d.dso - d.dod)
+                            : (d.zmin + (T)z * d.dv) * d.dso / (d.dso - d.dod);
                 int64_t idx = (int64_t)v * P + (int64_t)z * d.R + x;
                 out[idx] = in[idx] * d.dso / std::sqrt(x1 * x1 + x2 * x2 + x3 * x3);
             }
@@ -755,7 +759,7 @@ void fdk_bp_impl(const Geo<T> &d, const T *filt, T *vol, int v0, int nv) {
     std::vector<T> sn(nv), cs(nv);
     for (int v = 0; v < nv; ++v) view_angle(d, v0 + v, true, sn[v], cs[v]);
     const T dth = d.us_rate * pi_c<T>() / (T)180;
-    const double scale_den = (double)(d.us_rate * (T)d.nviews);
+    const T usn = d.us_rate * (T)d.nviews;
 #pragma omp parallel for collapse(2)
     for (int iz = 0; iz < d.Z; ++iz)
         for (int iy = 0; iy < d.N; ++iy)
@@ -786,7 +790,90 @@ void fdk_bp_impl(const Geo<T> &d, const T *filt, T *vol, int v0, int nv) {
                              an0 * ((T)1 - bn0) * pr[(int64_t)(an + 1) * d.R + bn] +
                              an0 * bn0 * pr[(int64_t)(an + 1) * d.R + bn + 1];
                     }
-                    acc = (T)((double)acc + (double)(t1 * t2 * dth) * 360.0 / scale_den);
+                    // temp1*temp2 * (us_rate*PI/180) * 360/(us_rate*Nviews), :60
+                    acc = acc + t1 * t2 * dth * (T)360 / usn;
+                }
+                vol[idx] = acc;
+            }
+}
+
+/* Multi-source FDK backprojection, one of the reference's three source
+ * combinations per view: 0 weights the sources whose stencil is on the
+ * detector by 1/(z - z_s)^2 and normalises (TB/FDK/kernel_FDK.cu:37-99);
+ * 1 takes the source nearest in z among those whose row lies on the
+ * detector (TB/FDK/OneRay/kernel_FDK.cu:37-113); 2 sums the sources
+ * (TB/FDK/EqualWeighting/kernel_FDK.cu:37-91, without its unsynchronised
+ * update).  Each view's combination is added to the volume
+ * (kernel_volume_divide, TB/FDK/kernel_FDK.cu:21-33). */
+template <typename T>
+void fdk_bp_ms_impl(const Geo<T> &d, const T *filt, T *vol, int v0, int nv, int mode) {
+    const int64_t P = (int64_t)d.R * d.ZP;
+    std::vector<T> sn(nv), cs(nv);
+    for (int v = 0; v < nv; ++v) view_angle(d, v0 + v, true, sn[v], cs[v]);
+    const T dth = d.us_rate * pi_c<T>() / (T)180;
+    const T usn = d.us_rate * (T)d.nviews;
+#pragma omp parallel for collapse(2)
+    for (int iz = 0; iz < d.Z; ++iz)
+        for (int iy = 0; iy < d.N; ++iy)
+            for (int ix = 0; ix < d.M; ++ix) {
+                int64_t idx = ((int64_t)iz * d.N + iy) * d.M + ix;
+                T x = d.bx + d.vx * (T)0.5 + (T)ix * d.vx;
+                T y = d.by + d.vy * (T)0.5 + (T)iy * d.vy;
+                T z = d.bz + d.vz * (T)0.5 + (T)iz * d.vz;
+                T acc = vol[idx];
+                for (int v = 0; v < nv; ++v) {
+                    const T c = cs[v], s = sn[v];
+                    const T t = x * c + y * s;
+                    const T sv = -x * s + y * c;
+                    const T t1 = d.dso * d.dso / ((d.dso - sv) * (d.dso - sv));
+                    // the view's interpolated value from source i (false: off the detector)
+                    auto sample = [&](int i, T &t2) -> bool {
+                        const T sz = source_z(d, i);
+                        T ksi = d.dso * (z - sz) / (d.dso - sv);
+                        ksi = (ksi * (d.dso - d.dod) / d.dso + sz - d.zmin) / d.dv;
+                        T p = d.dso * t / (d.dso - sv);
+                        p = (p * (d.dso - d.dod) / d.dso - d.ymin) / d.du;
+                        if (!(((T)0 <= ksi) && (ksi + 1 < (T)d.ZP) && ((T)0 <= p) && (p + 1 < (T)d.R))) return false;
+                        const int an = (int)std::floor(ksi), bn = (int)std::floor(p);
+                        const T an0 = ksi - (T)an, bn0 = p - (T)bn;
+                        const T *pr = filt + ((int64_t)v * d.NS + i) * P;
+                        t2 = ((T)1 - an0) * ((T)1 - bn0) * pr[(int64_t)d.R * an + bn] +
+                             ((T)1 - an0) * bn0 * pr[(int64_t)d.R * an + bn + 1] +
+                             an0 * ((T)1 - bn0) * pr[(int64_t)(an + 1) * d.R + bn] +
+                             an0 * bn0 * pr[(int64_t)(an + 1) * d.R + bn + 1];
+                        return true;
+                    };
+                    T upd = 0;
+                    if (mode == 0) {
+                        T wsum = 0;
+                        for (int i = 0; i < d.NS; ++i) {
+                            T t2;
+                            if (!sample(i, t2)) continue;
+                            const T dz = std::fabs(z - source_z(d, i));
+                            const T w = (T)1 / dz / dz;
+                            wsum += w;
+                            upd += t1 * t2 * dth * (T)360 / usn * w;
+                        }
+                        if (wsum > (T)0) upd /= wsum;
+                    } else if (mode == 1) {
+                        int sel = -1;
+                        for (int i = 0; i < d.NS; ++i) {
+                            const T sz = source_z(d, i);
+                            T ksi = d.dso * (z - sz) / (d.dso - sv);
+                            ksi = (ksi * (d.dso - d.dod) / d.dso + sz - d.zmin) / d.dv;
+                            if (((T)0 <= ksi) && (ksi + 1 < (T)d.ZP)) {
+                                if (sel < 0 || std::fabs(sz - z) < std::fabs(source_z(d, sel) - z)) sel = i;
+                            }
+                        }
+                        T t2;
+                        if (sel >= 0 && sample(sel, t2)) upd = t1 * t2 * dth * (T)360 / usn;
+                    } else {
+                        for (int i = 0; i < d.NS; ++i) {
+                            T t2;
+                            if (sample(i, t2)) upd += t1 * t2 * dth * (T)360 / usn;
+                        }
+                    }
+                    acc = acc + upd;
                 }
                 vol[idx] = acc;
             }
@@ -926,20 +1013,32 @@ void orc_fdk_backproject(const ct_geometry *g, int prec, const void *filt, void 
         fdk_bp_impl(derive<float>(g), (const float *)filt, (float *)vol, v0, nv),
         fdk_bp_impl(derive<double>(g), (const double *)filt, (double *)vol, v0, nv));
 }
-/* Reconstruction_FDK, V63/host_FDK.cpp:4-57: vol += FDK(proj). */
-void orc_fdk(const ct_geometry *g, int prec, const void *proj, void *vol, int window) {
-    const int64_t n = (int64_t)g->nviews * g->ncols * g->nrows;
+void orc_fdk_backproject_ms(const ct_geometry *g, int prec, const void *filt, void *vol, int v0, int nv, int mode) {
+    ORC_DISPATCH(prec,
+        fdk_bp_ms_impl(derive<float>(g), (const float *)filt, (float *)vol, v0, nv, mode),
+        fdk_bp_ms_impl(derive<double>(g), (const double *)filt, (double *)vol, v0, nv, mode));
+}
+/* Reconstruction_FDK, V63/host_FDK.cpp:4-57 (multi-source: TB/FDK/host_FDK.cpp:4-83
+ * with the source combination `mode`): vol += FDK(proj). */
+void orc_fdk_ms(const ct_geometry *g, int prec, const void *proj, void *vol, int window, int mode) {
+    const int ns = g->nsources < 1 ? 1 : g->nsources;
+    const int64_t n = (int64_t)g->nviews * ns * g->ncols * g->nrows;
     if (prec == 64) {
         std::vector<double> w(n), f(n);
         fdk_weight_impl(derive<double>(g), (const double *)proj, w.data());
-        ramp_impl(g, w.data(), f.data(), g->nviews, window);
-        fdk_bp_impl(derive<double>(g), f.data(), (double *)vol, 0, g->nviews);
+        ramp_impl(g, w.data(), f.data(), g->nviews * ns, window);
+        if (ns > 1) fdk_bp_ms_impl(derive<double>(g), f.data(), (double *)vol, 0, g->nviews, mode);
+        else fdk_bp_impl(derive<double>(g), f.data(), (double *)vol, 0, g->nviews);
     } else {
         std::vector<float> w(n), f(n);
         fdk_weight_impl(derive<float>(g), (const float *)proj, w.data());
-        ramp_impl(g, w.data(), f.data(), g->nviews, window);
-        fdk_bp_impl(derive<float>(g), f.data(), (float *)vol, 0, g->nviews);
+        ramp_impl(g, w.data(), f.data(), g->nviews * ns, window);
+        if (ns > 1) fdk_bp_ms_impl(derive<float>(g), f.data(), (float *)vol, 0, g->nviews, mode);
+        else fdk_bp_impl(derive<float>(g), f.data(), (float *)vol, 0, g->nviews);
     }
+}
+void orc_fdk(const ct_geometry *g, int prec, const void *proj, void *vol, int window) {
+    orc_fdk_ms(g, prec, proj, vol, window, 0);
 }
 
 } // extern "C"

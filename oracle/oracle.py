@@ -57,6 +57,8 @@ def _load() -> ctypes.CDLL:
         "orc_fft": ([P, P, I, I], None),
         "orc_fdk_backproject": ([G, I, P, P, I, I], None),
         "orc_fdk": ([G, I, P, P, I], None),
+        "orc_fdk_backproject_ms": ([G, I, P, P, I, I, I], None),
+        "orc_fdk_ms": ([G, I, P, P, I, I], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -244,9 +246,24 @@ def fdk_backproject(g: Geometry, filt: np.ndarray, vol: np.ndarray, view0: int =
     return v
 
 
-def fdk(g: Geometry, proj: np.ndarray, window: str = "ramp", prec: int = 32, vol: np.ndarray | None = None) -> np.ndarray:
+FDK_SOURCES = {"inverse_square": 0, "nearest": 1, "equal": 2}
+
+
+def fdk_backproject_ms(g: Geometry, filt: np.ndarray, vol: np.ndarray, mode: str = "inverse_square", view0: int = 0,
+                       nviews: int | None = None, prec: int = 32) -> np.ndarray:
+    """Multi-source FDK backprojection (any nsources, the TB kernels' path)."""
+    nv = g.nviews - view0 if nviews is None else nviews
+    f = _arr(filt, prec)
+    v = _arr(vol, prec).copy()
+    gc = g.to_c()
+    _lib.orc_fdk_backproject_ms(ctypes.byref(gc), prec, _p(f), _p(v), view0, nv, FDK_SOURCES[mode])
+    return v
+
+
+def fdk(g: Geometry, proj: np.ndarray, window: str = "ramp", prec: int = 32, vol: np.ndarray | None = None,
+        sources: str = "inverse_square") -> np.ndarray:
     p = _arr(proj, prec)
     v = np.zeros(g.vol_shape, dtype=_np_dtype(prec)) if vol is None else _arr(vol, prec).copy()
     gc = g.to_c()
-    _lib.orc_fdk(ctypes.byref(gc), prec, _p(p), _p(v), {"ramp": 0, "hann": 1}[window])
+    _lib.orc_fdk_ms(ctypes.byref(gc), prec, _p(p), _p(v), {"ramp": 0, "hann": 1}[window], FDK_SOURCES[sources])
     return v

@@ -379,3 +379,64 @@ def test_fdk_reconstructs_transposed_object():
     c_same = np.corrcoef(rec.ravel(), x.ravel())[0, 1]
     c_swap = np.corrcoef(rec.ravel(), x.transpose(0, 2, 1).ravel())[0, 1]
     assert c_swap > 0.8 and c_swap > c_same + 0.3
+
+
+MS_FDK = dict(nx=12, ny=12, nz=10, vx=1.0, ncols=20, nrows=14, du=1.3, nviews=6, dso=60.0, dod=-30.0,
+              nsources=3, src_z0=-4.0, src_dz=4.0)
+
+
+@pytest.mark.parametrize("mode", ["inverse_square", "nearest", "equal"])
+def test_multisource_fdk_one_source_is_fdk(mode):
+    """With one source at height 0 the three source combinations reduce to the
+    single-source FDK backprojection (the weights normalise to 1; the nearest
+    source is the only one)."""
+    g = Geometry(**{**MS_FDK, "nsources": 1, "src_z0": 0.0, "src_dz": 0.0})
+    f = np.random.default_rng(3).standard_normal((g.nviews, g.nrows, g.ncols))
+    v0 = np.zeros(g.vol_shape)
+    ref = orc.fdk_backproject(g, f, v0, prec=64)
+    got = orc.fdk_backproject_ms(g, f, v0, mode=mode, prec=64)
+    if mode == "inverse_square":
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-15)
+    else:
+        assert np.array_equal(got, ref)
+
+
+def test_multisource_fdk_combinations():
+    """Source combinations pinned by invariants on one view: "equal" is linear
+    in the per-source images; with all images 1 "inverse_square" gives the
+    largest single-source value (the weights normalise) wherever a source
+    reaches the voxel; with images c_i it is a convex combination of the
+    reaching sources' c_i; "nearest" returns c of the reaching source
+    nearest the voxel in z."""
+    g = Geometry(**{**MS_FDK, "nviews": 1})
+    shape = (1, g.nsources, g.nrows, g.ncols)
+    zero = np.zeros(g.vol_shape)
+    c = np.array([1.0, 10.0, 100.0])
+    single = []
+    for i in range(g.nsources):
+        f = np.zeros(shape)
+        f[0, i] = 1.0
+        single.append(orc.fdk_backproject_ms(g, f, zero, mode="equal", prec=64))   # t1*k where source i reaches
+    single = np.stack(single)
+    reach = single > 0
+    assert reach.any(axis=0).mean() > 0.5 and (reach.sum(axis=0) >= 2).any()
+    fc = np.broadcast_to(c[None, :, None, None], shape).copy()
+    equal = orc.fdk_backproject_ms(g, fc, zero, mode="equal", prec=64)
+    np.testing.assert_allclose(equal, np.tensordot(c, single, axes=1), rtol=1e-13)
+    ones = orc.fdk_backproject_ms(g, np.ones(shape), zero, mode="inverse_square", prec=64)
+    np.testing.assert_allclose(ones, single.max(axis=0), rtol=1e-13)
+    inv = orc.fdk_backproject_ms(g, fc, zero, mode="inverse_square", prec=64)
+    tk = single.max(axis=0)
+    lo = np.where(reach, c[:, None, None, None], np.inf).min(axis=0) * tk
+    hi = np.where(reach, c[:, None, None, None], -np.inf).max(axis=0) * tk
+    m = reach.any(axis=0)
+    assert np.all(inv[m] >= lo[m] * (1 - 1e-12)) and np.all(inv[m] <= hi[m] * (1 + 1e-12))
+    assert np.all(inv[~m] == 0)
+    near = orc.fdk_backproject_ms(g, fc, zero, mode="nearest", prec=64)
+    zc = -g.nz / 2 + 0.5 + np.arange(g.nz)                       # voxel-centre heights (vz = 1)
+    zs = g.src_z0 + g.src_dz * np.arange(g.nsources)
+    dist = np.abs(zs[:, None] - zc[None, :])[:, :, None, None]  # [source, z, 1, 1]
+    dist = np.where(reach, dist, np.inf)
+    pick = np.argmin(dist, axis=0)
+    want = np.where(m, c[pick] * tk, 0.0)
+    np.testing.assert_allclose(near, want, rtol=1e-13)
