@@ -727,6 +727,89 @@ __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict
     }
 }
 
+// Fan- and parallel-beam update with one detector row per slice (the
+// reference's setup for CT_style 1/2: Z_prj = ZETA, volumn_z =
+// Detector_pixel_x).  A voxel's rays all run in its own slice (the source
+// sits at the voxel's height, :715-724), so its chord through each footprint
+// column is the x/y slab interval times |S - P| in the plane, the same for
+// every slice of the (x, y) column, and only the correction row changes with
+// z.  A lane owns a column of PZR voxels: per view it forms the column range
+// (the reference's corner projection without magnification in z) and the
+// chords once, then each voxel adds chord x correction of its own row.  The
+// sums and the threshold are those of voxel_wl_fast with the z slab
+// unconstrained (the row lies at the voxel's height).
+constexpr int PZR = 16;
+template <typename SV>
+__global__ __launch_bounds__(256) void k_voxel_bp_planar(DGeo<float> d, const float *__restrict__ ang,
+                                                         SV *__restrict__ f, const float *__restrict__ corr, int first,
+                                                         int stride, int nv, float beta, int mode) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y * 4 + threadIdx.y;
+    const int iz0 = blockIdx.z * PZR;
+    if (ix >= d.M || iy >= d.N) return;
+    const int nz = min(PZR, d.Z - iz0);
+    // cylinder mask (:626-628)
+    const float hx = (float)d.M * 0.5f - 0.5f, hy = (float)d.N * 0.5f - 0.5f;
+    const float ma = (float)ix - hx - (float)d.moff, mb = (float)iy - hy;
+    if (ma * d.vx * ma * d.vx + mb * d.vy * mb * d.vy >= hx * d.vx * hy * d.vy) return;
+    const float cx = d.bx + d.vx * 0.5f + (float)ix * d.vx;
+    const float cy = d.by + d.vy * 0.5f + (float)iy * d.vy;
+    const float hvx = 0.5f * d.vx, hvy = 0.5f * d.vy;
+    const float xl = cx - hvx, xh = cx + hvx, yl = cy - hvy, yh = cy + hvy;
+    const float thr = d.vx * 1e-6f;
+    const int64_t P = (int64_t)d.R * d.ZP;
+    float W[PZR];
+#pragma unroll
+    for (int k = 0; k < PZR; ++k) W[k] = 0.0f;
+    float L = 0.0f;
+    for (int q = 0; q < nv; ++q) {
+        const int v = first + q * stride;
+        const float s = ang[2 * v], c = ang[2 * v + 1];
+        float minY = __builtin_inff(), maxY = -__builtin_inff();
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                const float px = ii ? xh : xl, py = jj ? yh : yl;
+                const float t = px * c + py * s;
+                const float sv = -px * s + py * c;
+                const float u = d.style == CT_PARALLEL ? sv : fmaf((t - d.dod) * frcp(d.dso - t), sv - d.sy, sv);
+                minY = fminf(minY, u); maxY = fmaxf(maxY, u);
+            }
+        int y0 = (int)floorf(fmaf(minY - d.ymin, d.idu, 0.5f));
+        int y1 = (int)floorf(fmaf(maxY - d.ymin, d.idu, 0.5f));
+        if ((y0 < 0 && y1 < 0) || (y0 > d.R - 1 && y1 > d.R - 1)) continue;
+        y0 = max(y0, 0); y1 = min(y1, d.R - 1);
+        const float sxw = d.sx * c - d.sy * s, syw = d.sx * s + d.sy * c;
+        const float *cq = corr + (int64_t)q * P + (int64_t)iz0 * d.R;
+        for (int i = y0; i <= y1; ++i) {
+            const float u = d.ymin + (float)i * d.du;
+            const float qx = d.dod * c - u * s, qy = d.dod * s + u * c;
+            const float ex = (d.style == CT_PARALLEL ? d.sx * c - u * s : sxw) - qx;
+            const float ey = (d.style == CT_PARALLEL ? d.sx * s + u * c : syw) - qy;
+            const float rx = frcp(ex), ry = frcp(ey);
+            const float ax0 = (xl - qx) * rx, ax1 = (xh - qx) * rx;
+            const float ay0 = (yl - qy) * ry, ay1 = (yh - qy) * ry;
+            const float span = fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)) - fmaxf(fminf(ax0, ax1), fminf(ay0, ay1));
+            const float len = fsqrt(fmaf(ex, ex, ey * ey)) * span;
+            if (!(span > 0.0f && len > thr)) continue;
+            L += len;
+#pragma unroll
+            for (int k = 0; k < PZR; ++k)
+                if (k < nz) W[k] = fmaf(len, cq[(int64_t)k * d.R + i], W[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PZR; ++k)
+        if (k < nz) bp_update(d, f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, W[k], L, beta, mode);
+}
+
+// Fan or parallel beam with the detector rows at the slice centres.
+static bool planar_eligible(const ct_context *ctx, const DGeo<float> &d) {
+    return ctx->math == CT_MATH_FAST && d.style != CT_CONE && d.ZP == d.Z &&
+           std::fabs((double)d.dv - (double)d.vz) <= 1e-6 * (double)d.vz && d.NS == 1;
+}
+
 // Host-side bound for k_voxel_bp_lds: every tile footprint must fit the
 // LDS window (LW columns, LH rows) and every voxel's bounding box NCMAX
 // columns.  With rho the largest distance from the axis of a point of a tile
@@ -812,6 +895,13 @@ static ct_status run_bp(ct_context *ctx, const ct_geometry &g, void *vol, const 
     if constexpr (std::is_same<T, float>::value) {
         if (lds_eligible(ctx, d, nv * d.NS))
             return run_bp_lds<SV, false>(ctx, d, ang, vol, corr, first, stride, nv, beta, mode);
+        if (planar_eligible(ctx, d)) {
+            ProfScope ps(ctx, CT_K_VOXEL_BP);
+            dim3 pgrid((d.M + 63) / 64, (d.N + 3) / 4, (d.Z + PZR - 1) / PZR);
+            hipLaunchKernelGGL((k_voxel_bp_planar<SV>), pgrid, block, 0, ctx->stream, d, ang, (SV *)vol,
+                               (const float *)corr, first, stride, nv, (float)beta, mode);
+            return launch_check(ctx, "voxel backprojection");
+        }
         if (ctx->math == CT_MATH_FAST) {
             ProfScope ps(ctx, CT_K_VOXEL_BP);
             hipLaunchKernelGGL((k_voxel_bp<T, SV, true>), grid, block, 0, ctx->stream, d, ang, (SV *)vol,
