@@ -108,6 +108,19 @@ template <> __device__ __forceinline__ float bload<__hip_bfloat16>(__amdgpu_buff
     return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
 }
 
+// The same with a wave-uniform part of the offset in an SGPR (soffset); the
+// range check applies to voffset + soffset.
+template <typename S> __device__ __forceinline__ float bload_so(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t so);
+template <> __device__ __forceinline__ float bload_so<float>(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, so, 0));
+}
+template <> __device__ __forceinline__ float bload_so<__half>(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t so) {
+    return __half2float(__builtin_bit_cast(__half, __builtin_amdgcn_raw_buffer_load_b16(r, off, so, 0)));
+}
+template <> __device__ __forceinline__ float bload_so<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t so) {
+    return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, __builtin_amdgcn_raw_buffer_load_b16(r, off, so, 0)));
+}
+
 inline size_t dtype_size(ct_dtype d) {
     switch (d) {
     case CT_F32: return 4;

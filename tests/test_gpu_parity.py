@@ -138,6 +138,22 @@ def test_sart_correction(ctx, name):
     assert rel(got, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("name", list(GEOMS))
+def test_sart_correction_batch(ctx, name):
+    """Five views: the batched path (z-quad volume, one workgroup per column
+    walking the column's shared x/y crossing list).  b = 2 Ax + 0.01 keeps
+    b - 0.9 Ax away from cancellation, so the bar measures the ray sums and
+    not the FP32 oracle's own rounding amplified by b - Ax."""
+    g = GEOMS[name]
+    x = phantom(g, seed=16, scale=0.3).numpy()
+    meas = orc.forward(g, x, prec=32, view0=0, nviews=5) * 2.0 + 0.01
+    ref = orc.sart_correction(g, x * 0.9, meas, view0=0, nviews=5, prec=32)
+    got = host(ctx.sart_correction(g, dev(x * 0.9), dev(meas), view0=0, nviews=5))
+    r = rel(got, ref)
+    print(f"{name}: batched correction rel L2 {r:.3e}")
+    assert r <= 1e-5
+
+
 @pytest.mark.parametrize("math_mode", ["exact", "fast"])
 @pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fine", "fan", "parallel"])
 @pytest.mark.parametrize("mode,nv", [(1, 1), (0, 1), (1, 3)])
