@@ -267,6 +267,11 @@ def main() -> int:
         torch.cuda.synchronize()
         vol = torch.zeros(g.vol_shape, device="cuda", dtype=sdt)
         ctx.sart_sweep(gs, vol, meas, beta=beta, subset=1)      # the sweep's kernels, as timed
+        # the same number of views from 90 degrees on: rays along y, which the
+        # correction walks ray by ray (the views above take the column walk)
+        from dataclasses import replace
+        g90 = replace(gs, angle0_deg=90.0)
+        ctx.sart_sweep(g90, vol, ctx.forward(g90, ball), beta=beta, subset=1)
         # FDK's kernels (weighted filter + backprojection) over the same views
         vol32 = torch.zeros(g.vol_shape, device="cuda")
         ctx.fdk(gs, meas.float(), vol32, window="hann")
