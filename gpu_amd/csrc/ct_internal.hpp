@@ -251,7 +251,7 @@ struct ProfScope {
 enum Slot {
     SLOT_CORR = 0,     // SART correction images
     SLOT_RED = 1,      // reduction partials
-    SLOT_FGP = 2,      // 7 FGP volumes
+    SLOT_FGP = 2,      // 9 FGP volumes (two dual sets, momentum copies)
     SLOT_FDK = 3,      // weighted + filtered projection stack
     SLOT_TMP_VOL = 4,  // adjoint test / power iteration volumes
     SLOT_TMP_VOL2 = 5,

@@ -392,37 +392,178 @@ __global__ __launch_bounds__(256) void k_fgp_L(const S *b, const T *r, const T *
     if (Lout) { if constexpr (std::is_same<S, double>::value) Lout[v] = L; else St<S>::st(Lout, v, L); }
 }
 
-// dual gradient step, unit-ball projection and momentum
-// (FGP_operator_LT_Pp, V63/kernel_FGP_Denoise_GPUx7.cu:27-88)
-template <typename T>
-__global__ __launch_bounds__(256) void k_fgp_LT(T *r, T *s, T *t, const T *L, T *o, T *p, T *q, int M, int N, int Z,
-                                                T g, T a12) {
-    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i >= M) return;
+// One FGP inner iteration in one pass (L never goes to memory).  The dual
+// fields are double-buffered: the pass reads r, s, t of the previous
+// iteration from one set and writes the next iteration's to the other, so a
+// workgroup can read its neighbours' old values at the tile borders whatever
+// their progress.  A workgroup owns a 128 x 8 (x, y) tile and marches a chunk
+// of z slices: it stages the old r, s, t of the next slice in LDS with a
+// one-voxel border, forms L of that slice on the tile and its right column
+// and bottom row (the neighbours the dual step needs), then updates the
+// current slice.  Per voxel the arithmetic is that of k_fgp_L followed by the
+// dual gradient step, unit-ball projection and momentum of FGP_operator_LT_Pp
+// (V63/kernel_FGP_Denoise_GPUx7.cu:27-88).  HBM traffic ~13.8 words per voxel (13 for b, the old duals, o,
+// p, q read and the new duals, o, p, q written; the rest is the staged
+// border) against 18 for the two-kernel form.
+constexpr int FGX = 128, FGY = 8;            // tile (threads)
+constexpr int FEX = FGX + 2, FEY = FGY + 2;  // staged extent: one voxel of border
+constexpr int FLX = FGX + 1, FLY = FGY + 1;  // L extent: tile + right column + bottom row
+constexpr int FGZ = 64;                      // slices per workgroup
+
+template <typename S, typename T>
+__global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b, const T *__restrict__ rA,
+                                                        const T *__restrict__ sA, const T *__restrict__ tA,
+                                                        T *__restrict__ rB, T *__restrict__ sB, T *__restrict__ tB,
+                                                        T *__restrict__ o, T *__restrict__ p, T *__restrict__ q, int M,
+                                                        int N, int Z, T lambda, T g, T a12) {
+    // staged old duals: three slices in flight (written one step ahead of
+    // their first reader, so no barrier guards the write); L: two slices
+    __shared__ T sr[3][FEY * FEX], ss[3][FEY * FEX], stt[3][FEY * FEX];
+    __shared__ T sL[2][FLY * FLX];
+    constexpr int NT = FGX * FGY;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FGX + tx;
+    // x/y neighbours on one XCD: their border lines meet in its L2
+    const int3 tile = xcd_tile();
+    const int i0 = tile.x * FGX, j0 = tile.y * FGY;
+    const int k0 = tile.z * FGZ, k1 = min(k0 + FGZ, Z);
+    if (k0 >= k1) return;
     const int64_t MN = (int64_t)M * N;
-    const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
-    T tr = r[v], ts = s[v], tt = t[v];
-    const T Lv = L[v];
-    if (i < M - 1) tr = tr + a12 * (Lv - L[v + 1]);
-    if (j < N - 1) ts = ts + a12 * (Lv - L[v + M]);
-    if (k < Z - 1) tt = tt + a12 * (Lv - L[v + MN]);
-    const T nrm = sqrt(tr * tr + ts * ts + tt * tt);
-    const T den = nrm > (T)1 ? nrm : (T)1;
-    if (i < M - 1) { const T on = tr / den; r[v] = on + g * (on - o[v]); o[v] = on; }
-    if (j < N - 1) { const T pn = ts / den; s[v] = pn + g * (pn - p[v]); p[v] = pn; }
-    if (k < Z - 1) { const T qn = tt / den; t[v] = qn + g * (qn - q[v]); q[v] = qn; }
+    // this thread's staged and L entries (the same in every slice): offset
+    // in the slice, or -1 outside the volume / past the extent
+    int soff[2], loff[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int e = tid + u * NT;
+        soff[u] = -1; loff[u] = -1;
+        if (e < FEX * FEY) {
+            const int ly = e / FEX - 1, lx = e - (ly + 1) * FEX - 1;
+            const int i = i0 + lx, j = j0 + ly;
+            if (i >= 0 && i < M && j >= 0 && j < N) soff[u] = j * M + i;
+        }
+        if (e < FLX * FLY) {
+            const int ly = e / FLX, lx = e - ly * FLX;
+            const int i = i0 + lx, j = j0 + ly;
+            if (i < M && j < N) loff[u] = j * M + i;
+        }
+    }
+    // every load of a slice is issued one step before its use: the staged
+    // duals (extended tile, zero outside the volume), b on the L extent and
+    // the owner's o, p, q
+    T gr[2], gs[2], gt[2], gb[2];
+    auto fetch_stage = [&](int k) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            gr[u] = 0; gs[u] = 0; gt[u] = 0;
+            if (soff[u] >= 0) {
+                const int64_t v = (int64_t)k * MN + soff[u];
+                gr[u] = rA[v]; gs[u] = sA[v]; gt[u] = tA[v];
+            }
+        }
+    };
+    auto put_stage = [&](int k) {
+        const int bf = k % 3;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + u * NT;
+            if (e < FEX * FEY) { sr[bf][e] = gr[u]; ss[bf][e] = gs[u]; stt[bf][e] = gt[u]; }
+        }
+    };
+    auto fetch_b = [&](int k) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            gb[u] = 0;
+            if (loff[u] >= 0) {
+                const int64_t v = (int64_t)k * MN + loff[u];
+                if constexpr (std::is_same<S, double>::value) gb[u] = b[v]; else gb[u] = ldf(b, v);
+            }
+        }
+    };
+    // L of slice k on the tile + right column + bottom row (k_fgp_L)
+    auto formL = [&](int k, const T (&bv)[2]) {
+        const int bf = k % 3, pf = (k + 2) % 3;   // slices k and k - 1
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + u * NT;
+            if (e >= FLX * FLY) continue;
+            const int ly = e / FLX, lx = e - ly * FLX;
+            const int i = i0 + lx, j = j0 + ly;
+            T L = 0;
+            if (i < M && j < N) {
+                const int c = (ly + 1) * FEX + (lx + 1);
+                const T rm = i > 0 ? sr[bf][c - 1] : (T)0;
+                const T sm = j > 0 ? ss[bf][c - FEX] : (T)0;
+                const T tm = k > 0 ? stt[pf][c] : (T)0;
+                const T tl = (sr[bf][c] - rm) + (ss[bf][c] - sm) + (stt[bf][c] - tm);
+                L = bv[u] - lambda * tl;
+            }
+            sL[k & 1][e] = L;
+        }
+    };
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own = i < M && j < N;
+    T po = 0, pp = 0, pq = 0;
+    auto fetch_own = [&](int k) {
+        if (own) {
+            const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+            po = o[v]; pp = p[v]; pq = q[v];
+        }
+    };
+    // prologue: slices k0 - 1 and k0 staged, L(k0) formed, slice k0 + 1's
+    // duals, b of k0 + 1 and the owner's o, p, q of k0 in flight
+    if (k0 > 0) { fetch_stage(k0 - 1); put_stage(k0 - 1); }
+    fetch_stage(k0); put_stage(k0);
+    fetch_b(k0);
+    __syncthreads();
+    formL(k0, gb);
+    if (k0 + 1 < Z) { fetch_stage(k0 + 1); fetch_b(k0 + 1); }
+    fetch_own(k0);
+    for (int k = k0; k < k1; ++k) {
+        const bool nx = k + 1 < Z;
+        if (nx) put_stage(k + 1);
+        const T co = po, cp = pp, cq = pq;
+        T bn[2] = {gb[0], gb[1]};
+        if (k + 2 < Z && k + 1 < k1) { fetch_stage(k + 2); fetch_b(k + 2); }
+        if (k + 1 < k1) fetch_own(k + 1);
+        __syncthreads();   // slice k + 1 staged; L(k) complete
+        if (nx) formL(k + 1, bn);
+        __syncthreads();   // L(k + 1) complete
+        if (!own) continue;
+        // dual step, projection and momentum (FGP_operator_LT_Pp)
+        const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
+        const int c = (ty + 1) * FEX + (tx + 1), l = ty * FLX + tx;
+        const int bf = k % 3;
+        const T r0 = sr[bf][c], s0 = ss[bf][c], t0 = stt[bf][c];
+        T tr = r0, ts = s0, tt = t0;
+        const T Lv = sL[k & 1][l];
+        if (i < M - 1) tr = tr + a12 * (Lv - sL[k & 1][l + 1]);
+        if (j < N - 1) ts = ts + a12 * (Lv - sL[k & 1][l + FLX]);
+        if (k < Z - 1) tt = tt + a12 * (Lv - sL[(k + 1) & 1][l]);
+        const T nrm = sqrt(tr * tr + ts * ts + tt * tt);
+        const T den = nrm > (T)1 ? nrm : (T)1;
+        if (i < M - 1) { const T on = tr / den; rB[v] = on + g * (on - co); o[v] = on; }
+        else rB[v] = r0;
+        if (j < N - 1) { const T pn = ts / den; sB[v] = pn + g * (pn - cp); p[v] = pn; }
+        else sB[v] = s0;
+        if (k < Z - 1) { const T qn = tt / den; tB[v] = qn + g * (qn - cq); q[v] = qn; }
+        else tB[v] = t0;
+    }
 }
 
 template <typename S, typename T>
 static ct_status run_fgp(ct_context *ctx, int M, int N, int Z, void *vol, double lambda, int niter) {
     const int64_t V = (int64_t)M * N * Z;
     ct_status st;
-    T *w = (T *)slot(ctx, SLOT_FGP, sizeof(T) * V * 7, &st);
+    // (the step keeps in-slice offsets in 32 bits)
+    if ((int64_t)M * N >= (1ll << 31)) return fail(ctx, CT_ERR_OVERFLOW, "fgp: slice of 2^31 voxels or more");
+    // two sets of duals (read one, write the other) and the momentum copies
+    T *w = (T *)slot(ctx, SLOT_FGP, sizeof(T) * V * 9, &st);
     if (!w) return st;
-    T *r = w, *s = w + V, *t = w + 2 * V, *o = w + 3 * V, *p = w + 4 * V, *q = w + 5 * V, *L = w + 6 * V;
-    st = launch_fill(ctx, w, sizeof(T) == 8 ? CT_F64 : CT_F32, V * 6, 0.0);
+    T *dual[2][3] = {{w, w + V, w + 2 * V}, {w + 6 * V, w + 7 * V, w + 8 * V}};
+    T *o = w + 3 * V, *p = w + 4 * V, *q = w + 5 * V;
+    st = launch_fill(ctx, w, sizeof(T) == 8 ? CT_F64 : CT_F32, V * 9, 0.0);
     if (st != CT_OK) return st;
     dim3 b(256), gr((M + 255) / 256, N, Z);
+    dim3 fb(FGX, FGY), fg((M + FGX - 1) / FGX, (N + FGY - 1) / FGY, (Z + FGZ - 1) / FGZ);
     const T lam = (T)lambda;
     const T a12 = (T)1 / (lam * (T)12);
     T tk1 = 1;
@@ -430,8 +571,9 @@ static ct_status run_fgp(ct_context *ctx, int M, int N, int Z, void *vol, double
         const T tk = ((T)1 + std::sqrt((T)1 + (T)4 * tk1 * tk1)) / (T)2;
         const T g = (tk1 - (T)1) / tk;
         ProfScope ps(ctx, CT_K_FGP);
-        hipLaunchKernelGGL((k_fgp_L<S, T>), gr, b, 0, ctx->stream, (const S *)vol, r, s, t, (S *)nullptr, L, M, N, Z, lam);
-        hipLaunchKernelGGL((k_fgp_LT<T>), gr, b, 0, ctx->stream, r, s, t, (const T *)L, o, p, q, M, N, Z, g, a12);
+        T *const *A = dual[it & 1], *const *B = dual[(it + 1) & 1];
+        hipLaunchKernelGGL((k_fgp_step<S, T>), fg, fb, 0, ctx->stream, (const S *)vol, (const T *)A[0],
+                           (const T *)A[1], (const T *)A[2], B[0], B[1], B[2], o, p, q, M, N, Z, lam, g, a12);
         tk1 = tk;
     }
     hipLaunchKernelGGL((k_fgp_L<S, T>), gr, b, 0, ctx->stream, (const S *)vol, o, p, q, (S *)vol, (T *)nullptr, M, N, Z, lam);

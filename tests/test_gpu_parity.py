@@ -289,8 +289,9 @@ def test_fista_momentum(ctx):
         assert rel(host(P), p_ref) == 0.0
 
 
-@pytest.mark.parametrize("shape", [(7, 9, 11), (16, 24, 32)])
+@pytest.mark.parametrize("shape", [(7, 9, 11), (16, 24, 32), (131, 19, 70)])
 def test_tv_fgp(ctx, shape):
+    """(131, 19, 70): several z chunks and partial x/y tiles of the fused step."""
     rng = np.random.default_rng(4)
     b = (rng.standard_normal(shape) * 0.01 + 0.02).astype(np.float32)
     ref = orc.tv_fgp(b, 0.004, 20, prec=32)
