@@ -230,8 +230,9 @@ def main() -> int:
                          "oracle), FP32, FP16 and BF16; prints per-iteration RMSE / relative L2 against FP64 and "
                          "the phantom as one JSON line, then exits")
     ap.add_argument("--pmc-sample", type=int, default=0,
-                    help="counter-collection mode: a SART sweep and an FDK over the first N views at full shape "
-                         "plus a 1 GiB device copy (calibration); exits without the JSON line")
+                    help="counter-collection mode: SART sweeps over N views from 0 and from 90 degrees, an FDK over "
+                         "the first N views and two TV-FGP inner iterations at full shape, plus a 1 GiB device "
+                         "copy (calibration); exits without the JSON line")
     ap.add_argument("--pmc-dtype", choices=("f32", "f16"), default="f32",
                     help="storage type of the counter-collection sweep")
     args = ap.parse_args()
@@ -275,6 +276,8 @@ def main() -> int:
         # FDK's kernels (weighted filter + backprojection) over the same views
         vol32 = torch.zeros(g.vol_shape, device="cuda")
         ctx.fdk(gs, meas.float(), vol32, window="hann")
+        # two TV-FGP inner iterations (one fused step each) on that volume
+        ctx.tv_fgp(vol32, 1e-3, 2)
         torch.cuda.synchronize()
         log(f"pmc sample done: {args.pmc_sample} views")
         return 0
