@@ -150,6 +150,41 @@ def style_figures(ctx, g, x32) -> dict:
     return out
 
 
+def multisource_figures(ctx) -> dict:
+    """The multi-source autograd workload at its own size (48 point sources 3
+    mm apart on the axis, 720 views of 1024 x 128 at 0.5 mm, 256^3 at 0.75 mm,
+    BF16 storage, FP32 accumulation): ray sums then the matched adjoint on 4
+    batches of 2 views (96 images each) spread over the orbit, and the time
+    one Adam iteration's forward + gradient projection would take at that
+    rate."""
+    import torch
+    from gpu_amd.geometry import multisource_256
+    from gpu_amd.phantom import phantom
+
+    gm = multisource_256()
+    # the seeded phantom scaled to +-80 mm inside the +-96 mm volume
+    x = phantom(gm, seed=0, scale=2.0, device="cuda").to(torch.bfloat16)
+    nbatch, bviews = 4, 2
+    # warm-up (first launches of these kernel instantiations)
+    ctx.adjoint(gm, ctx.forward(gm, x, view0=0, nviews=1), view0=0, nviews=1)
+    ctx.profile(True, reset=True)
+    for j in range(nbatch):
+        v0 = j * (gm.nviews // nbatch)
+        pj = ctx.forward(gm, x, view0=v0, nviews=bviews)
+        ctx.adjoint(gm, pj, view0=v0, nviews=bviews)
+    torch.cuda.synchronize()
+    pa = ctx.profile_read()
+    ctx.profile(False)
+    nimg = nbatch * bviews * gm.nsources
+    f_ms, a_ms = pa["forward"][0] / nimg, pa["adjoint"][0] / nimg
+    rays = gm.nrows * gm.ncols
+    return {"grays": round(rays / ((f_ms + a_ms) * 1e-3) / 1e9, 4),
+            "forward_ms_per_image": round(f_ms, 5), "adjoint_ms_per_image": round(a_ms, 5),
+            "est_s_per_adam_iteration": round((f_ms + a_ms) * gm.nviews * gm.nsources / 1e3, 2),
+            "sample": f"{nbatch} batches of {bviews} views x {gm.nsources} sources, 256^3 BF16, 720 views of "
+                      "1024 x 128; one image = one (view, source) pair"}
+
+
 def io_figures(ctx, g, meas32, x32) -> dict:
     """The callers either side of the path, on the bench's own data: 64 view
     files read back into HBM (from the page cache: the files were just
@@ -384,6 +419,8 @@ def main() -> int:
     result["beam_styles"] = style_figures(ctx, g, x32)
     log("beam styles: " + ", ".join(f"{k} {v['grays']} GRays/s" for k, v in result["beam_styles"].items()
                                      if isinstance(v, dict)))
+    result["multisource_pair"] = multisource_figures(ctx)
+    log(f"multi-source pair: {result['multisource_pair']['grays']} GRays/s")
     result["io"] = io_figures(ctx, g, meas32, x32)
     log("files: " + ", ".join(f"{k} {v}" for k, v in result["io"].items() if k != "note"))
 

@@ -1103,7 +1103,7 @@ __device__ __forceinline__ void slab(float k, float b0, int i0, int n, float &in
 constexpr int AZR = 8;          // z-run per lane
 constexpr int ALW = 80;         // staged footprint columns
 constexpr int ALP = ALW + 8;    // row pitch (zero columns for the NC overrun)
-constexpr int ALH = 32;         // staged footprint rows
+constexpr int ALH = 48;         // staged footprint rows (48 sources 3 mm apart need 36)
 constexpr int ANC = 8;          // bounding-box columns per voxel
 
 template <int NC>
@@ -1501,9 +1501,7 @@ static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const 
     const float *ang = (const float *)angle_table(ctx, g, false, false, &st);
     if (!ang) return st;
     DGeo<float> d = derive<float>(g);
-    const int nimg = nv * d.NS;
     ProfScope ps(ctx, CT_K_FORWARD);
-    (void)nimg;
     const int nq = col_walk_nq(d);
     const int64_t P = (int64_t)d.R * d.ZP;
     by_run(g, nq > 0, first, stride, nv, [&](int q0, int n, bool cw) {

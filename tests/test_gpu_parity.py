@@ -13,7 +13,7 @@ import pytest
 import torch
 
 from gpu_amd import core
-from gpu_amd.geometry import Geometry, adjoint_64, plain_sart_128, repo_scale_512
+from gpu_amd.geometry import Geometry, adjoint_64, multisource_256, plain_sart_128, repo_scale_512
 from gpu_amd.phantom import phantom
 from oracle import oracle as orc
 
@@ -288,6 +288,32 @@ def test_adjoint_matches_oracle(ctx, name):
     ref32 = orc.adjoint(g, y, prec=32)
     got32 = host(ctx.adjoint(g, dev(y)))
     assert rel(got32, ref32) <= 1e-5
+
+
+def test_adjoint_multisource_full_size(ctx):
+    """The multi-source autograd geometry itself (48 sources 3 mm apart, 1024 x
+    128 detector, 256^3 at 0.75 mm): the sources' spread needs 36 footprint
+    rows per 16-voxel tile, which the gather adjoint's 48-row window holds.
+    One view (48 images) of N(0, 1) data against the FP64 oracle.  A voxel's
+    segment is the difference of two FP32 plane crossings of size ~0.5 (6e-8
+    rounding) over an alpha length of ~5e-4 (0.75 mm of a 1.5 m ray), so each
+    segment carries ~1.2e-4 relative error.  The forward's sums telescope it
+    away; a voxel's adjoint sum of unrelated rays of cancelling N(0, 1) data
+    does not.  Hence the 2e-4 bar, against the FP64 oracle.  The matched
+    pair itself passes the FP32 dot test over two views."""
+    g = multisource_256().with_views(1)
+    rng = np.random.default_rng(9)
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    ref64 = orc.adjoint(g, y.astype(np.float64), prec=64)
+    ref32 = orc.adjoint(g, y, prec=32)
+    got32 = host(ctx.adjoint(g, dev(y)))
+    e_gpu, e_plain = rel(got32, ref64), rel(ref32, ref64)
+    lhs, rhs = ctx.adjoint_test(multisource_256().with_views(2), dtype=torch.float32, seed=5)
+    d = abs(lhs - rhs) / abs(lhs)
+    print(f"multi-source 256^3: adjoint vs FP64 {e_gpu:.3e} (plain FP32 {e_plain:.3e}, "
+          f"GPU vs plain {rel(got32, ref32):.3e}), dot test {d:.3e}")
+    assert e_gpu <= 2e-4
+    assert d <= 1e-6
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-6)])
