@@ -334,7 +334,7 @@ typedef enum ct_kernel_id {
     CT_K_FORWARD = 0,       /* Siddon ray sum / SART correction */
     CT_K_VOXEL_BP = 1,      /* voxel-driven backprojection + update */
     CT_K_ADJOINT = 2,       /* matched adjoint scatter */
-    CT_K_FGP = 3,           /* one TV-FGP inner iteration (both kernels) */
+    CT_K_FGP = 3,           /* one TV-FGP inner iteration (one fused launch) */
     CT_K_FISTA = 4,         /* FISTA momentum */
     CT_K_FDK_BP = 5,        /* FDK backprojection */
     CT_K_RAMP = 6,          /* ramp filter */
