@@ -1,6 +1,6 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
 
-Bars (from the north-star contract): plain FP32 path <= 1e-5 relative L2 on
+Bars: plain FP32 path <= 1e-5 relative L2 on
 projections and <= 1e-4 on reconstructions; FP64 paths are held to 1e-12;
 the matched adjoint passes the dot-product test to 1e-12 (FP64) / 1e-6 (FP32).
 """
