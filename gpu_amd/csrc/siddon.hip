@@ -781,31 +781,36 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     }
     const unsigned nbytes = MN * (unsigned)ZQ4 * 4u * (unsigned)ES;
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(fq, nbytes);
+    // this thread's staged quads (the same in every batch): segment k of the
+    // batch and quad qi of its window (sk = -1: none), and where they go
     const int nst = COL_B * NQ;   // quads staged per batch (at most 2 * COL_ROWS)
+    int sk[2], sq[2], so[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const int sidx = tid + p * COL_ROWS;
+        const int k = sidx / NQ;
+        sk[p] = sidx < nst ? k : -1;
+        sq[p] = sidx - k * NQ;
+        so[p] = k * COL_W + 4 * sq[p];
+    }
+    // a quad outside the volume reads as zeros: its index is clamped to -1 or
+    // ZQ4, whose offsets lie past the buffer's end (the volume is under 2 GiB)
     float4 sreg[2];
     auto issue = [&](int b) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            const int sidx = tid + p * COL_ROWS;
-            sreg[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (sidx < nst) {
-                const int k = sidx / NQ, qi = sidx - k * NQ;
-                const uint2 en = ls[b * COL_B + k];
-                const int qz = (int)en.y + qi;
-                if ((unsigned)qz < (unsigned)ZQ4) sreg[p] = load_quad<SV>(rs, en.x + (unsigned)qz * QS);
+            if (sk[p] >= 0) {
+                const uint2 en = ls[b * COL_B + sk[p]];
+                const int qz = min(max((int)en.y + sq[p], -1), ZQ4);
+                sreg[p] = load_quad<SV>(rs, en.x + (unsigned)qz * QS);
             }
         }
     };
     auto commit = [&](auto bufc) {
         constexpr int buf = decltype(bufc)::value;
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int sidx = tid + p * COL_ROWS;
-            if (sidx < nst) {
-                const int k = sidx / NQ, qi = sidx - k * NQ;
-                *reinterpret_cast<float4 *>(stg + ((buf * COL_B + k) * COL_W + 4 * qi)) = sreg[p];
-            }
-        }
+        for (int p = 0; p < 2; ++p)
+            if (sk[p] >= 0) *reinterpret_cast<float4 *>(stg + (buf * COL_B * COL_W + so[p])) = sreg[p];
     };
     __syncthreads();   // the list is complete
     const int nb = KP / COL_B;
