@@ -623,8 +623,10 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
 // the part after the crossing reads the next z voxel.  The segments are those
 // of the per-ray walk (same alphas, same voxels; a tie there is one step, here
 // a zero-length segment), and the sums keep its FP32-batch / FP64-total
-// accumulation.  Per segment ~16 VALU, two LDS reads and no per-lane gather,
-// against ~25 VALU and one scattered gather per step of the per-ray walk.
+// accumulation.  Per segment ~18 VALU (staging included), two LDS reads and
+// no per-lane gather, against ~25 VALU and one scattered gather per step of
+// the per-ray walk.  The launchers use it for views whose rays run along x
+// (see col_walk_view).
 constexpr int COL_ROWS = 256;   // rays per workgroup: rows of one column
 constexpr int COL_B = 8;        // list segments per staged batch
 constexpr int COL_NQ = 48;      // largest staged window, in z-quads
