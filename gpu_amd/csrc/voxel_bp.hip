@@ -427,9 +427,30 @@ __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const flo
     }
 }
 
+// Stochastic rounding of an updated voxel to 16-bit storage.  A SART step
+// adds ~1e-4 of a voxel's value, a fraction of an FP16 ulp; rounding to
+// nearest loses it with a bias that accumulates over iterations.  The dropped
+// significand bits (13 for FP16, 16 for BF16) get a dither, then the value is
+// truncated: it rounds away from zero with probability equal to the dropped
+// fraction, so the stored value is the exact one in expectation.  The dither
+// is the low significand bits of the voxel's own sums W and L (sums of ~10^2
+// products that change with every view), which is one instruction per
+// voxel, and keeps runs reproducible.
+template <typename SV> __device__ __forceinline__ void st_sr(SV *p, int64_t i, float v, uint32_t r) {
+    St<SV>::st(p, i, v);
+}
+template <> __device__ __forceinline__ void st_sr<__half>(__half *p, int64_t i, float v, uint32_t r) {
+    p[i] = __float2half_rz(__uint_as_float(__float_as_uint(v) + (r & 0x1FFFu)));
+}
+template <> __device__ __forceinline__ void st_sr<__hip_bfloat16>(__hip_bfloat16 *p, int64_t i, float v, uint32_t r) {
+    const uint32_t u = __float_as_uint(v) + (r & 0xFFFFu);
+    p[i] = __builtin_bit_cast(__hip_bfloat16, (unsigned short)(u >> 16));
+}
+
 // SART update of one voxel from its accumulated W = sum l*c and L = sum l
 // (V63/kernel_IterativeRecon_universal.cu:799-806; TB threshold
-// TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772)
+// TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772).  16-bit volumes store the
+// update with stochastic rounding.
 template <typename SV>
 __device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__ f, int64_t idx, float W, float L,
                                           float beta, int mode) {
@@ -438,7 +459,7 @@ __device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__
     else upd = (double)L > (double)d.vx * 1e-3;
     if (!upd) return;
     if (mode == 0) St<SV>::st(f, idx, W);
-    else St<SV>::st(f, idx, ldf(f, idx) + beta * W / L);
+    else st_sr<SV>(f, idx, ldf(f, idx) + beta * W / L, __float_as_uint(W) ^ __float_as_uint(L));
 }
 
 // ONE: the launch backprojects a single image (one view, one source), so each
