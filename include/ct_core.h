@@ -168,7 +168,10 @@ ct_status ct_sart_correction(ct_context *ctx, const ct_geometry *g,
 /* Voxel-driven backprojection (backprj_ray_driven_3d_kernel,
  * V63/kernel_IterativeRecon_universal.cu:571-810), summed over the nviews
  * correction images: mode 1: x += beta*W/L where L >= 1e-6*vx (SART update);
- * mode 0: x = W where L >= 1e-6*vx (unchanged elsewhere). */
+ * mode 0: x = W where L >= 1e-6*vx (unchanged elsewhere).  In fast mode a
+ * CT_F16/CT_BF16 volume stores the mode-1 update with stochastic rounding
+ * (a dither taken from W and L, so runs are reproducible): small updates are
+ * kept in expectation instead of being rounded away. */
 ct_status ct_backproject_voxel(ct_context *ctx, const ct_geometry *g,
                                void *vol, ct_dtype vol_dtype,
                                const void *corr,
