@@ -57,6 +57,12 @@ template <typename T> static void angle_host(const ct_geometry &g, int j, bool f
     if (!fdk) th = -th;
     s = std::sin(th); c = std::cos(th);
 }
+// cos of view j's projector angle (FP32, as the kernels' angle table holds it)
+float view_cos(const ct_geometry &g, int j) {
+    float s, c;
+    angle_host<float>(g, j, false, s, c);
+    return c;
+}
 
 const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fdk, ct_status *st) {
     std::vector<double> key = {(double)g.nviews, g.angle_step_deg, g.angle0_deg, g.angle_shift_deg,

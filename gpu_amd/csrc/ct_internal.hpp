@@ -273,6 +273,7 @@ void *slot(ct_context *ctx, int which, size_t bytes, ct_status *st);
 const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fdk, ct_status *st);
 ct_status check_ptr(ct_context *ctx, const void *p, ct_dtype dt, const char *name);
 ct_status validate_geometry(ct_context *ctx, const ct_geometry *g);
+float view_cos(const ct_geometry &g, int j);   // projector angle of view j (the kernels' table)
 // makes the context's device current for the calls of one entry point
 struct DeviceGuard {
     explicit DeviceGuard(ct_context *ctx) { hipSetDevice(ctx->device); }

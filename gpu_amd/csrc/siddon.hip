@@ -1376,11 +1376,7 @@ static void launch_col(ct_context *ctx, const DGeo<float> &d, int nq, const floa
 // 40 degrees, 0.67 vs 0.44 ms along y.  Views whose central ray is within
 // ~41 degrees of the x axis (|cos| >= 0.75) take the column walk; by_run
 // splits a launch's views into runs of one choice.
-static bool col_walk_view(const ct_geometry &g, int v) {
-    float s, c;
-    if (ct_view_angle(&g, v, 0, &s, &c) != CT_OK) return false;
-    return std::fabs(c) >= 0.75f;
-}
+static bool col_walk_view(const ct_geometry &g, int v) { return std::fabs(view_cos(g, v)) >= 0.75f; }
 template <typename F> static void by_run(const ct_geometry &g, bool allow, int first, int stride, int nv, F &&launch) {
     int q0 = 0;
     bool cur = allow && nv > 0 && col_walk_view(g, first);
