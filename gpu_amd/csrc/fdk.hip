@@ -500,8 +500,10 @@ ct_status launch_fdk_weight(ct_context *ctx, const ct_geometry &g, const void *i
 // ------------------------------------------------------- backprojection
 // kernel_FDK_reconstruction (V63/FDK/kernel_FDK.cu:16-62) over a batch of views:
 // one voxel per lane accumulates all views in a register and writes once.
-// The reference's per-view weight t1*t2*dtheta*360/(us_rate*Nviews) is applied
-// with the FP32 factor dtheta*360/(us_rate*Nviews).
+// The reference's per-view term temp1*temp2*dtheta*360.0/(us_rate*Nviews) is
+// formed as it is there: the FP32 product temp1*temp2*dtheta, then the
+// double literal 360.0 makes the rest of the expression and the += double,
+// and the sum is stored back as float (V63/FDK/kernel_FDK.cu:60).
 template <typename T, typename SP, typename SV>
 __global__ __launch_bounds__(256) void k_fdk_bp(DGeo<T> d, const T *__restrict__ ang, const SP *__restrict__ filt,
                                                 SV *__restrict__ vol, int view0, int nv, T dth, T usn) {
@@ -540,8 +542,11 @@ __global__ __launch_bounds__(256) void k_fdk_bp(DGeo<T> d, const T *__restrict__
             t2 = ((T)1 - an0) * ((T)1 - bn0) * p00 + ((T)1 - an0) * bn0 * p01 + an0 * ((T)1 - bn0) * p10 +
                  an0 * bn0 * p11;
         }
-        // temp1*temp2 * (us_rate*PI/180) * 360/(us_rate*Nviews), V63/FDK/kernel_FDK.cu:60
-        acc = acc + t1 * t2 * dth * (T)360 / usn;
+        // temp1*temp2 * (us_rate*PI/180) * 360.0/(us_rate*Nviews), V63/FDK/kernel_FDK.cu:60
+        if constexpr (std::is_same<T, float>::value)
+            acc = (float)((double)acc + (double)(t1 * t2 * dth) * 360.0 / (double)usn);
+        else
+            acc = acc + t1 * t2 * dth * (T)360 / usn;
     }
     if constexpr (std::is_same<SV, double>::value) vol[idx] = acc; else St<SV>::st(vol, idx, (float)acc);
 }

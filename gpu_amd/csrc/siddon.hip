@@ -356,6 +356,13 @@ template <int ES, bool ZQ = false> struct FastWalk {
         RayState<float> r = ray_setup(d, s, c, col, row, src);
         if (!r.hit) return false;
         const unsigned M = d.M, N = d.N, Z = d.Z, MN = M * N;
+        // The reference's walk begins in the voxel its FP32 setup names and
+        // stops before the first segment when that voxel lies outside the
+        // grid, so the ray comes out as 0 (V63/kernel_IterativeRecon_universal.cu
+        // :271-284, :297-303, :554-563).  This happens at exact ties, where
+        // the entry point rounds onto the outer side of an edge plane; such
+        // rays are empty here too, in the forward and the matched adjoint.
+        if ((unsigned)r.vi >= M || (unsigned)r.vj >= N || (unsigned)r.vk >= Z) return false;
         const PlaneAxis X = plane_axis(d.vx, d.ivx, d.bx, r.x1x, r.ixd, r.sgx, d.vx * 1e-6f > fabsf(r.x1x - r.x2x));
         const PlaneAxis Y = plane_axis(d.vy, d.ivy, d.by, r.x1y, r.iyd, r.sgy, d.vy * 1e-6f > fabsf(r.x1y - r.x2y));
         const PlaneAxis Zx = plane_axis(d.vz, d.ivz, d.bz, r.x1z, r.izd, r.sgz, d.vz * 1e-6f > fabsf(r.x1z - r.x2z));

@@ -435,16 +435,24 @@ __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const flo
 // fraction, so the stored value is the exact one in expectation.  The dither
 // is the low significand bits of the voxel's own sums W and L (sums of ~10^2
 // products that change with every view), which is one instruction per
-// voxel, and keeps runs reproducible.
+// voxel, and keeps runs reproducible.  In the FP16 subnormal range
+// (|v| < 2^-14) the conversion drops more than 13 bits, so the dither there is
+// added in float as u * 2^-24 (one subnormal ulp) with u in [0, 1) from the
+// same bits; NaN and infinities are stored as they are.
 template <typename SV> __device__ __forceinline__ void st_sr(SV *p, int64_t i, float v, uint32_t r) {
     St<SV>::st(p, i, v);
 }
 template <> __device__ __forceinline__ void st_sr<__half>(__half *p, int64_t i, float v, uint32_t r) {
-    p[i] = __float2half_rz(__uint_as_float(__float_as_uint(v) + (r & 0x1FFFu)));
+    const float a = fabsf(v);
+    const float wn = __uint_as_float(__float_as_uint(v) + (r & 0x1FFFu));
+    const float ws = v + __builtin_copysignf((float)(r & 0xFFFFu) * 0x1p-40f, v);
+    p[i] = __float2half_rz(a < 0x1p-14f ? ws : (a <= 65504.0f ? wn : v));
 }
 template <> __device__ __forceinline__ void st_sr<__hip_bfloat16>(__hip_bfloat16 *p, int64_t i, float v, uint32_t r) {
     const uint32_t u = __float_as_uint(v) + (r & 0xFFFFu);
-    p[i] = __builtin_bit_cast(__hip_bfloat16, (unsigned short)(u >> 16));
+    // a NaN keeps a quiet-NaN pattern instead of carrying into the sign bit
+    const unsigned short h = v == v ? (unsigned short)(u >> 16) : (unsigned short)0x7FC0u;
+    p[i] = __builtin_bit_cast(__hip_bfloat16, h);
 }
 
 // SART update of one voxel from its accumulated W = sum l*c and L = sum l
@@ -744,7 +752,10 @@ __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict
     if (mode == 0) St<SV>::st(f, idx, W);
     else {
         const T x = (T)ldf(f, idx);
-        St<SV>::st(f, idx, x + beta * W / L);
+        // fast mode stores 16-bit updates with stochastic rounding, as the
+        // LDS-staged and planar kernels do (exact mode rounds to nearest)
+        if constexpr (FAST) st_sr<SV>(f, idx, x + beta * W / L, __float_as_uint((float)W) ^ __float_as_uint((float)L));
+        else St<SV>::st(f, idx, x + beta * W / L);
     }
 }
 

@@ -537,8 +537,10 @@ void fgp_L(int M, int N, int Z, const T *b, const T *r, const T *s, const T *t, 
             }
 }
 
+/* duals (optional): the projected dual fields (o, p, q) after the last
+ * iteration, for the unit-ball check of the tests */
 template <typename T>
-void fgp_impl(int M, int N, int Z, const T *b, T *out, T lambda, int niter) {
+void fgp_impl(int M, int N, int Z, const T *b, T *out, T lambda, int niter, T *duals = nullptr) {
     const int64_t V = (int64_t)M * N * Z, MN = (int64_t)M * N;
     if (lambda == 0) { std::memcpy(out, b, sizeof(T) * V); return; }
     std::vector<T> r(V, 0), s(V, 0), t(V, 0), o(V, 0), p(V, 0), q(V, 0), L(V, 0);
@@ -566,6 +568,11 @@ void fgp_impl(int M, int N, int Z, const T *b, T *out, T lambda, int niter) {
         tk1 = tk;
     }
     fgp_L(M, N, Z, b, o.data(), p.data(), q.data(), out, lambda);     /* :83 */
+    if (duals) {
+        std::memcpy(duals, o.data(), sizeof(T) * V);
+        std::memcpy(duals + V, p.data(), sizeof(T) * V);
+        std::memcpy(duals + 2 * V, q.data(), sizeof(T) * V);
+    }
 }
 
 /* D: x -> (x_v - x_{v+e}) inside, 0 on the last plane; D^T: p -> p_v - p_{v-e}. */
@@ -790,8 +797,10 @@ void fdk_bp_impl(const Geo<T> &d, const T *filt, T *vol, int v0, int nv) {
                              an0 * ((T)1 - bn0) * pr[(int64_t)(an + 1) * d.R + bn] +
                              an0 * bn0 * pr[(int64_t)(an + 1) * d.R + bn + 1];
                     }
-                    // temp1*temp2 * (us_rate*PI/180) * 360/(us_rate*Nviews), :60
-                    acc = acc + t1 * t2 * dth * (T)360 / usn;
+                    // temp1*temp2 * (us_rate*PI/180.0f) * 360.0/(us_rate*Nviews), :60:
+                    // the FP32 product, then the double literal 360.0 takes the
+                    // rest of the expression and the += to double
+                    acc = (T)((double)acc + (double)(t1 * t2 * dth) * 360.0 / (double)usn);
                 }
                 vol[idx] = acc;
             }
@@ -953,6 +962,11 @@ void orc_tv_fgp(int prec, int M, int N, int Z, const void *b, void *out, double 
     ORC_DISPATCH(prec,
         fgp_impl(M, N, Z, (const float *)b, (float *)out, (float)lambda, niter),
         fgp_impl(M, N, Z, (const double *)b, (double *)out, lambda, niter));
+}
+void orc_tv_fgp_duals(int prec, int M, int N, int Z, const void *b, void *out, void *duals, double lambda, int niter) {
+    ORC_DISPATCH(prec,
+        fgp_impl(M, N, Z, (const float *)b, (float *)out, (float)lambda, niter, (float *)duals),
+        fgp_impl(M, N, Z, (const double *)b, (double *)out, lambda, niter, (double *)duals));
 }
 void orc_tv_D(int prec, int M, int N, int Z, const void *x, void *r, void *s, void *t) {
     ORC_DISPATCH(prec,

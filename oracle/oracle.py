@@ -45,6 +45,7 @@ def _load() -> ctypes.CDLL:
         "orc_ray_segments": ([G, I, I, I, I, I, P, P, I], I),
         "orc_fista_momentum": ([I, P, P, P, I64, D], None),
         "orc_tv_fgp": ([I, I, I, I, P, P, D, I], None),
+        "orc_tv_fgp_duals": ([I, I, I, I, P, P, P, D, I], None),
         "orc_tv_D": ([I, I, I, I, P, P, P, P], None),
         "orc_tv_Dt": ([I, I, I, I, P, P, P, P], None),
         "orc_tv_norm": ([I, I, I, I, P], D),
@@ -162,6 +163,16 @@ def tv_fgp(b: np.ndarray, lam: float, niter: int = 20, prec: int = 32) -> np.nda
     nz, ny, nx = B.shape
     _lib.orc_tv_fgp(prec, nx, ny, nz, _p(B), _p(out), lam, niter)
     return out
+
+
+def tv_fgp_duals(b: np.ndarray, lam: float, niter: int, prec: int = 64):
+    """FGP result and its projected dual fields (o, p, q) after niter iterations."""
+    B = _arr(b, prec)
+    out = np.zeros_like(B)
+    duals = np.zeros((3, *B.shape), dtype=B.dtype)
+    nz, ny, nx = B.shape
+    _lib.orc_tv_fgp_duals(prec, nx, ny, nz, _p(B), _p(out), _p(duals), lam, niter)
+    return out, duals
 
 
 def tv_D(x: np.ndarray, prec: int = 64):

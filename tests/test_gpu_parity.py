@@ -160,11 +160,13 @@ def test_forward_full_size(ctx, angle0):
     views from 0 degrees (rays along x: the column walk, four 256-row
     workgroups per column) and from 90 degrees (the per-ray walk).
 
-    At this size the reference's FP32 walk returns 0 for a few rays that do
-    cross the volume (exact ties of its plane indices; the FP64 walk drops
-    others).  Exact mode reproduces the reference bit for bit, dropped rays
-    included.  Fast mode returns those rays' line integrals: it is held to the
-    1e-5 bar on every other ray, and the dropped rays must stay a handful."""
+    At this size the reference's FP32 setup starts a few rays that do cross
+    the volume outside the grid (exact ties at the first view of each
+    orbit: 1 ray at 0 degrees, 19 at 90), and its walk returns 0 for them.
+    Both modes reproduce that: exact mode bit for bit, fast mode within the
+    1e-5 bar over every ray.  Where the reference returns 0, fast mode stays
+    below 1e-5 (0 for the tie rays, whose line integrals are 0.04-1.56; at
+    most a value times a sub-threshold chord for grazing rays)."""
     g = replace(repo_scale_512(), angle0_deg=angle0).with_views(4)
     x = phantom(g, seed=0, scale=2.5, device="cuda").cpu().numpy()
     ref = orc.forward(g, x, prec=32)
@@ -176,13 +178,12 @@ def test_forward_full_size(ctx, angle0):
     finally:
         ctx.set_math("fast")
     assert np.array_equal(ex, ref.astype(np.float64))
-    dropped = (ref == 0) & (got > 1e-3)
-    keep = ~dropped
-    r = rel(got[keep], ref[keep])
-    print(f"512^3 from {angle0} deg: fast rel L2 {r:.3e} over {keep.sum()} rays; "
-          f"{dropped.sum()} rays the reference returns as 0")
+    r = rel(got, ref)
+    zeroed = ref == 0
+    print(f"512^3 from {angle0} deg: fast rel L2 {r:.3e} over all {ref.size} rays; "
+          f"{int(zeroed.sum())} rays zero in the reference, max fast value there {got[zeroed].max():.3e}")
     assert r <= 1e-5
-    assert dropped.sum() <= 16
+    assert got[zeroed].max() <= 1e-5
 
 
 def test_sart_sweep_full_size(ctx):

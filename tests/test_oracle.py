@@ -440,3 +440,321 @@ def test_multisource_fdk_combinations():
     pick = np.argmin(dist, axis=0)
     want = np.where(m, c[pick] * tk, 0.0)
     np.testing.assert_allclose(near, want, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- FDK pins
+
+def cylinder_projections(g, radius, height, mu):
+    """Analytic line integrals (FP64) of a centred uniform cylinder
+    x^2 + y^2 <= radius^2, |z| <= height/2 of attenuation mu, on the forward
+    model's rays (detector pixel -> source)."""
+    out = np.zeros((g.nviews, g.nrows, g.ncols))
+    u = -g.du * (g.ncols / 2 - 0.5) + np.arange(g.ncols) * g.du
+    dv = g.dv or g.du
+    v = -dv * (g.nrows / 2 - 0.5) + np.arange(g.nrows) * dv
+    for j in range(g.nviews):
+        s, c = orc.view_angle(g, j, prec=64)
+        x1x, x1y = g.dod * c - u * s, g.dod * s + u * c
+        dx, dy = g.dso * c - x1x, g.dso * s - x1y
+        a = dx * dx + dy * dy
+        b = 2 * (x1x * dx + x1y * dy)
+        disc = b * b - 4 * a * (x1x ** 2 + x1y ** 2 - radius ** 2)
+        hit = disc > 0
+        sq = np.sqrt(np.where(hit, disc, 0.0))
+        a0, a1 = (-b - sq) / (2 * a), (-b + sq) / (2 * a)
+        dz = -v[:, None]
+        t0, t1 = (-height / 2 - v[:, None]) / dz, (height / 2 - v[:, None]) / dz
+        lo = np.maximum(a0[None, :], np.minimum(t0, t1))
+        hi = np.minimum(a1[None, :], np.maximum(t0, t1))
+        out[j] = np.where(hit[None, :], np.maximum(hi - lo, 0.0) * np.sqrt(a[None, :] + dz * dz), 0.0) * mu
+    return out
+
+
+@pytest.mark.parametrize("dso,dod", [(100.0, -60.0), (1000.0, -500.0)])
+def test_fdk_uniform_cylinder_amplitude(dso, dod):
+    """FDK (V63/host_FDK.cpp:4-57) of analytic projections of a uniform
+    cylinder over 360 views returns mu, flat, in the central slices: the mean
+    inside 12 mm is within 2% of mu (the discrete ramp's known DC deficit is
+    ~1.4%) and every 3 mm ring is within 0.4% of that mean.  A dropped or
+    doubled 0.5 in the ramp (V63/FDK/FFT_1d_GPU.h:266-267) or a wrong view
+    weight (V63/FDK/kernel_FDK.cu:60) moves the mean by 2x or more; a wrong
+    power of DSO/(DSO - s) (:36) tilts the rings by 1.6-3% at DSO = 100 mm,
+    against 0.23% with the right one."""
+    g = Geometry(nx=48, ny=48, nz=4, vx=1.0, ncols=160, nrows=48, du=0.6 * (dso - dod) / dso, nviews=360,
+                 dso=dso, dod=dod)
+    mu = 0.02
+    rec = orc.fdk(g, cylinder_projections(g, 18.0, 60.0, mu), "ramp", prec=64)
+    mid = rec[1:3]
+    yy, xx = np.meshgrid(np.arange(48) - 23.5, np.arange(48) - 23.5, indexing="ij")
+    r = np.hypot(xx, yy)
+    mean = mid[:, r < 12].mean()
+    rings = np.array([mid[:, (r >= a) & (r < a + 3)].mean() for a in (0, 3, 6, 9, 12)])
+    print(f"DSO {dso}: mean/mu {mean / mu:.4f}, rings/mean {np.round(rings / mean, 4)}")
+    assert abs(mean / mu - 1) < 0.02
+    assert np.all(np.abs(rings / mean - 1) < 0.004)
+
+
+def test_fdk_backprojection_affine_view():
+    """One filtered view that is affine in (row, column), q = a + b col + c row,
+    backprojects to DSO^2/(DSO - s)^2 (a + b p + c ksi) 2 pi / N_views exactly
+    (V63/FDK/kernel_FDK.cu:33-60): bilinear interpolation reproduces an affine
+    function, so the test fixes the stencil's orientation (b != c catches
+    swapped row and column fractions), the distance weight and the view factor
+    dtheta * 360 / (us_rate * N_views) = 2 pi / N_views.  The detector
+    coordinates p and ksi are computed here from the geometry's definition:
+    voxel centre, +theta rotation, magnification to the detector plane."""
+    g = Geometry(nx=14, ny=12, nz=10, vx=1.1, ncols=40, nrows=30, du=0.9, nviews=7, dso=80.0, dod=-50.0,
+                 angle_step_deg=23.0, angle0_deg=11.0)
+    a, b, c = 0.7, 0.013, -0.029
+    rows, cols = np.meshgrid(np.arange(g.nrows), np.arange(g.ncols), indexing="ij")
+    for view in (0, 3, 6):
+        filt = np.zeros((g.nviews, g.nrows, g.ncols))
+        filt[view] = a + b * cols + c * rows
+        got = orc.fdk_backproject(g, filt, np.zeros(g.vol_shape), prec=64)
+        th = math.radians(view * 23.0 + 11.0)
+        sn, cs = math.sin(th), math.cos(th)
+        zc = (np.arange(g.nz) - g.nz / 2 + 0.5) * g.vx
+        yc = (np.arange(g.ny) - g.ny / 2 + 0.5) * g.vx
+        xc = (np.arange(g.nx) - g.nx / 2 + 0.5) * g.vx
+        Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+        t = X * cs + Y * sn
+        s = -X * sn + Y * cs
+        mag = (g.dso - g.dod) / (g.dso - s)
+        p = (t * mag + g.du * (g.ncols / 2 - 0.5)) / g.du
+        ksi = (Z * mag + g.du * (g.nrows / 2 - 0.5)) / g.du
+        inside = (ksi >= 0) & (ksi + 1 < g.nrows) & (p >= 0) & (p + 1 < g.ncols)
+        want = np.where(inside, (g.dso / (g.dso - s)) ** 2 * (a + b * p + c * ksi) * 2 * math.pi / g.nviews, 0.0)
+        assert inside.mean() > 0.5
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------- FGP pins
+
+def np_D(x):
+    """Forward differences x_v - x_{v+e}, zero on the last plane (numpy, FP64)."""
+    r, s, t = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    r[:, :, :-1] = x[:, :, :-1] - x[:, :, 1:]
+    s[:, :-1, :] = x[:, :-1, :] - x[:, 1:, :]
+    t[:-1, :, :] = x[:-1, :, :] - x[1:, :, :]
+    return r, s, t
+
+
+def np_Dt(r, s, t):
+    out = r.copy() + s + t
+    out[:, :, 1:] -= r[:, :, :-1]
+    out[:, 1:, :] -= s[:, :-1, :]
+    out[1:, :, :] -= t[:-1, :, :]
+    return out
+
+
+def unit_ball(r, s, t):
+    n = np.maximum(1.0, np.sqrt(r * r + s * s + t * t))
+    return r / n, s / n, t / n
+
+
+@pytest.mark.parametrize("lam", [0.5, 1e-3])
+def test_fgp_first_iteration_closed_form(lam):
+    """From zero duals one FGP step (V63/kernel_FGP_Denoise_GPUx7.cu:2-88,
+    V63/host_FGP_Denoise_GPUx7.c:64-86) is the projected gradient step of the
+    dual of min ||x - b||^2 + 2 lam TV(x) with step 1/(12 lam):
+    x1 = b - lam D^T P(D b / (12 lam)), P the per-voxel projection onto the
+    unit ball.  lam = 0.5 keeps every |Db|/(12 lam) < 1 (x1 = b - D^T D b / 12,
+    a Laplacian step independent of lam), lam = 1e-3 saturates the
+    projection.  A wrong step constant, lambda scaling or projection fails."""
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((6, 7, 8)) * 0.1
+    x1 = orc.tv_fgp(b, lam, 1, prec=64)
+    r, s, t = np_D(b)
+    want = b - lam * np_Dt(*unit_ball(r / (12 * lam), s / (12 * lam), t / (12 * lam)))
+    np.testing.assert_allclose(x1, want, rtol=1e-12, atol=1e-14)
+    if lam == 0.5:
+        np.testing.assert_allclose(x1, b - np_Dt(r, s, t) / 12, rtol=1e-12, atol=1e-14)
+
+
+def prox_tv_pdhg(b, lam, iters=8000):
+    """argmin_x ||x - b||^2 + 2 lam sum_v |(Dx)_v| by the primal-dual method of
+    Chambolle and Pock (2011, algorithm 2: G strongly convex, accelerated),
+    an independent solver for the FGP fixed point."""
+    x = b.copy()
+    xb = x.copy()
+    y = [np.zeros_like(b) for _ in range(3)]
+    L2 = 12.0                    # ||D||^2 <= 12 in 3-D
+    tau, sigma, gam = 1.0, 1.0 / 12.0, 2.0
+    for _ in range(iters):
+        d = np_D(xb)
+        y = [yi + sigma * di for yi, di in zip(y, d)]
+        n = np.maximum(1.0, np.sqrt(y[0] ** 2 + y[1] ** 2 + y[2] ** 2) / (2 * lam))
+        y = [yi / n for yi in y]
+        xn = (x - tau * np_Dt(*y) + 2 * tau * b) / (1 + 2 * tau)
+        theta = 1 / math.sqrt(1 + 2 * gam * tau)
+        tau, sigma = theta * tau, sigma / theta
+        xb = xn + theta * (xn - x)
+        x = xn
+    assert tau * sigma * L2 <= 1.0 + 1e-9
+    return x
+
+
+def test_fgp_converges_to_tv_prox():
+    """Run long enough, FGP returns argmin ||x - b||^2 + 2 lam TV(x) with the
+    isotropic forward-difference TV of the reference (zero on the last
+    planes): it agrees with an independent primal-dual solver to 1e-4 of the
+    volume (both converge like 1/k; the solvers' own distance to the
+    minimiser is ~2e-5) and reaches the same objective to 1e-7.  A different
+    lambda convention (||x - b||^2 + lam TV, or a lost factor in the step)
+    moves the fixed point by more than 1% here.  The FP32 instantiation
+    follows FP64."""
+    rng = np.random.default_rng(11)
+    b = np.zeros((8, 9, 10))
+    b[2:6, 2:7, 3:8] = 1.0
+    b += rng.normal(0, 0.1, b.shape)
+    lam = 0.05
+    ref = prox_tv_pdhg(b, lam)
+    got = orc.tv_fgp(b, lam, 3000, prec=64)
+    obj = lambda x: float(((x - b) ** 2).sum() + 2 * lam * np.sqrt(sum(d * d for d in np_D(x))).sum())
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"FGP(3000) vs primal-dual: rel {err:.2e}, objectives {obj(got):.12f} {obj(ref):.12f}")
+    assert err < 1e-4
+    assert abs(obj(got) - obj(ref)) <= 1e-7 * obj(ref)
+    assert np.linalg.norm(orc.tv_fgp(b, 2 * lam, 3000, prec=64) - ref) / np.linalg.norm(ref) > 1e-2
+    assert np.linalg.norm(orc.tv_fgp(b, lam / 2, 3000, prec=64) - ref) / np.linalg.norm(ref) > 1e-2
+    x32 = orc.tv_fgp(b, lam, 200, prec=32)
+    x64 = orc.tv_fgp(b, lam, 200, prec=64)
+    assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-5
+
+
+def test_fgp_duals_in_unit_ball_and_acceleration():
+    """After every iteration the projected duals (o, p, q) lie in the unit
+    ball per voxel (V63/kernel_FGP_Denoise_GPUx7.cu:67-85), and the momentum
+    makes FGP converge faster than the plain projected gradient (the same
+    iteration with gamma = 0, emulated from single steps): after 30
+    iterations FGP's objective gap is below a third of the gradient's."""
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal((5, 6, 7)) * 0.3
+    lam = 0.08
+    for n in (1, 2, 3, 5, 8, 13, 30):
+        _, duals = orc.tv_fgp_duals(b, lam, n, prec=64)
+        nrm = np.sqrt((duals ** 2).sum(axis=0))
+        assert nrm.max() <= 1.0 + 1e-12, n
+        assert nrm.max() > 0.99        # the projection is active for this lambda
+    ref = prox_tv_pdhg(b, lam)
+    obj = lambda x: float(((x - b) ** 2).sum() + 2 * lam * np.sqrt(sum(d * d for d in np_D(x))).sum())
+    gap_fgp = obj(orc.tv_fgp(b, lam, 30, prec=64)) - obj(ref)
+    # plain projected gradient on the dual: p <- P(p + D(b - lam D^T p)/(12 lam))
+    p = [np.zeros_like(b) for _ in range(3)]
+    for _ in range(30):
+        L = b - lam * np_Dt(*p)
+        d = np_D(L)
+        p = list(unit_ball(*[pi + di / (12 * lam) for pi, di in zip(p, d)]))
+    gap_gp = obj(b - lam * np_Dt(*p)) - obj(ref)
+    print(f"objective gap after 30: FGP {gap_fgp:.3e}, projected gradient {gap_gp:.3e}")
+    assert 0 <= gap_fgp < gap_gp / 3
+
+
+def cpu_fdnoise(b, lam, niter):
+    """numpy restatement of the reference's CPU FGP, F_Dnoise
+    (V63/host_FGP_Denoise_CPU.h:242-347) with F_OperatorL (:57-120),
+    F_OperatorLT (:134-153) and F_ProjectionP (:157-238) on its compact dual
+    arrays X: Z*Y*(X-1), Y: Z*(Y-1)*X, Z: (Z-1)*Y*X, including its
+    out-of-step reads on the last x, y and z planes (reads past the end of an
+    array return 0 here)."""
+    NZ, NY, NX = b.shape
+    nX, nY, nZ = NZ * NY * (NX - 1), NZ * (NY - 1) * NX, (NZ - 1) * NY * NX
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+
+    def rd(a, idx):
+        a1 = np.concatenate([a, np.zeros(4 * NX * NY + 8)])
+        return np.where(idx >= 0, a1[np.clip(idx, 0, a1.size - 1)], 0.0)
+
+    def opL(X, Y, Z):
+        xi = (k * NY + j) * (NX - 1) + i
+        yi = (k * (NY - 1) + j) * NX + i
+        zi = (k * NY + j) * NX + i
+        A = rd(X, xi) - np.where(i > 0, rd(X, xi - 1), 0)
+        A += rd(Y, yi) - np.where(j > 0, rd(Y, ((k * (NY - 1) + (j - 1)) * NX + i)), 0)
+        A += rd(Z, zi) - np.where(k > 0, rd(Z, (((k - 1) * NY + j) * NX + i)), 0)
+        return A
+
+    def opLT(A):
+        X = (A[:, :, :-1] - A[:, :, 1:]).ravel()
+        Y = (A[:, :-1, :] - A[:, 1:, :]).ravel()
+        Z = (A[:-1, :, :] - A[1:, :, :]).ravel()
+        return X, Y, Z
+
+    def projP(X1, Y1, Z1):
+        X, Y, Z = np.zeros(nX), np.zeros(nY), np.zeros(nZ)
+        for kk in range(NZ):
+            for jj in range(NY):
+                for ii in range(NX):
+                    xi = (kk * NY + jj) * (NX - 1) + ii
+                    yi = (kk * (NY - 1) + jj) * NX + ii
+                    zi = (kk * NY + jj) * NX + ii
+                    hx, hy, hz = ii < NX - 1, jj < NY - 1, kk < NZ - 1
+                    q = (X1[xi] ** 2 if hx else 0) + (Y1[yi] ** 2 if hy else 0) + (Z1[zi] ** 2 if hz else 0)
+                    den = max(1.0, math.sqrt(q))
+                    if hx:
+                        X[xi] = X1[xi] / den
+                    if hy:
+                        Y[yi] = Y1[yi] / den
+                    if hz:
+                        Z[zi] = Z1[zi] / den
+        return X, Y, Z
+
+    X0, Y0, Z0 = np.zeros(nX), np.zeros(nY), np.zeros(nZ)
+    Xm, Ym, Zm = np.zeros(nX), np.zeros(nY), np.zeros(nZ)
+    X, Y, Z = Xm, Ym, Zm
+    t = 1.0
+    for _ in range(niter):
+        tmp = b - lam * opL(X0, Y0, Z0)
+        X1, Y1, Z1 = opLT(tmp)
+        X, Y, Z = projP(X0 + X1 / (12 * lam), Y0 + Y1 / (12 * lam), Z0 + Z1 / (12 * lam))
+        tp1 = (1 + math.sqrt(1 + 4 * t * t)) / 2
+        X0, Y0, Z0 = X + (X - Xm) * (t - 1) / tp1, Y + (Y - Ym) * (t - 1) / tp1, Z + (Z - Zm) * (t - 1) / tp1
+        Xm, Ym, Zm = X, Y, Z
+        t = tp1
+    return b - lam * opL(X, Y, Z)
+
+
+def test_fgp_gpux7_equals_cpu_fgp_in_the_interior():
+    """The oracle follows the GPUx7 kernels.  The reference's own CPU FGP
+    (F_Dnoise) is the same algorithm on compact dual arrays, except that its
+    divergence reads out of step on the last x, y and z planes; those reads
+    reach one more voxel inward per iteration.  Away from the last planes the
+    two agree to rounding, which ties the oracle to the reference's second
+    statement of the algorithm; on the last planes they differ."""
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal((12, 13, 14)) * 0.2
+    lam, n = 0.03, 3
+    cpu = cpu_fdnoise(b, lam, n)
+    got = orc.tv_fgp(b, lam, n, prec=64)
+    m = 2 * n + 2
+    inner = (slice(0, 12 - m), slice(0, 13 - m), slice(0, 14 - m))
+    np.testing.assert_allclose(got[inner], cpu[inner], rtol=1e-12, atol=1e-14)
+    assert np.abs(got - cpu).max() > 1e-6
+
+
+# ------------------------------------------------ multi-source voxel backprojection pins
+
+def test_multisource_voxel_chords_and_threshold():
+    """Multi-source voxel-driven backprojection (TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:546-775):
+    with c = 1 on every image of a view, W summed over the view's sources
+    equals A^T 1 of that view built from the independent Siddon traversal at
+    every voxel inside the reconstruction cylinder, so the source culling
+    (:618-627) drops no source whose rays reach the voxel.  With c = kappa the
+    update is beta*kappa where L > 1e-3 vx and nothing elsewhere (:772)."""
+    g = small_geom(nx=10, ny=10, nz=12, nsources=4, src_z0=-7.5, src_dz=5.0, nrows=10)
+    ii, jj = np.meshgrid(np.arange(g.nx), np.arange(g.ny), indexing="xy")
+    hx, hy = g.nx * 0.5 - 0.5, g.ny * 0.5 - 0.5
+    mask = np.broadcast_to(((ii - hx) ** 2 + (jj - hy) ** 2) < hx * hy, g.vol_shape)
+    for view in (0, 2, 5):
+        ones = np.ones((1, g.nsources, g.nrows, g.ncols))
+        W = orc.backproject_voxel(g, np.zeros(g.vol_shape), ones, view0=view, nviews=1, beta=1.0, mode=0, prec=64)
+        proj = np.zeros((g.nviews, g.nsources, g.nrows, g.ncols))
+        proj[view] = 1.0
+        at1 = orc.adjoint(g, proj, prec=64)
+        thr = at1 > 1e-3 * g.vx
+        assert (thr & mask).sum() > 200
+        np.testing.assert_allclose(W[mask & thr], at1[mask & thr], rtol=1e-9, atol=1e-9)
+        upd = orc.backproject_voxel(g, np.zeros(g.vol_shape), 0.6 * ones, view0=view, nviews=1, beta=0.5,
+                                    mode=1, prec=64)
+        keep = mask & ((at1 > 1.001e-3 * g.vx) | (at1 < 0.999e-3 * g.vx))
+        np.testing.assert_allclose(upd[keep], np.where(thr, 0.3, 0.0)[keep], rtol=1e-12, atol=1e-15)
