@@ -66,7 +66,8 @@ def run_path(ctx, g, meas, dtype, steps, warmup, beta, lam, niter):
 
 def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
     """Per-iteration RMSE / relative L2 of the FP32, FP16 and BF16 paths against
-    the FP64 path (same data, same schedule) and against the phantom."""
+    the FP64 path (same data, same schedule, from zero) and against the
+    phantom.  Returns the record and the seconds per iteration of each path."""
     import torch
     paths = {"f64": torch.float64, "f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}
     state = {}
@@ -74,7 +75,8 @@ def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
         meas = meas32 if dt in (torch.float32, torch.float64) else meas32.to(dt)
         state[k] = [torch.zeros(g.vol_shape, device="cuda", dtype=dt), torch.zeros(g.vol_shape, device="cuda", dtype=dt),
                     1.0, meas, 0.0]
-    out = {"workload": f"512^3 / 1024 views SART(subset 1, beta {beta}) + TV-FGP({niter}, {lam}) + FISTA",
+    out = {"workload": f"512^3 / 1024 views SART(subset 1, beta {beta:.6f}) + TV-FGP({niter}, {lam}) + FISTA, "
+                       "from zero",
            "reference": "f64 path on the GPU (FP64 instantiation of the same kernels)", "iterations": []}
     truth = x_true.double()
     tn = float(torch.linalg.vector_norm(truth))
@@ -101,26 +103,87 @@ def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
                       "rel_l2_vs_phantom": float(torch.linalg.vector_norm(xk - truth)) / tn}
         out["iterations"].append(rec)
     out["seconds_per_iteration"] = {k: state[k][4] / iters for k in paths}
-    print(json.dumps(out), flush=True)
-    return 0
+    del state
+    torch.cuda.empty_cache()
+    return out
 
 
-def cpu_baseline(g, meas_host, nviews, beta):
-    """Oracle SART correction + voxel update for the first `nviews` views of the
-    same sweep, FP32, OpenMP over all host cores."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(g, meas_host, beta, lam, niter, sweep_views, full=False):
+    """The oracle (OpenMP restatement of the reference kernels, FP32, -O3, IEEE)
+    on the host cores, on samples of the same workloads:
+
+    * the 512^3 / 1024-view SART sweep's first `sweep_views` views (correction +
+      voxel update), extrapolated to the whole sweep and iteration (labelled);
+    * TV-FGP inner iterations at 512^3 and one FISTA momentum step;
+    * FDK (Hann) of 4 views at 512^3;
+    * the 128^3 / 180-view SART workload: one SART iteration (subsets of 20)
+      by default, the full ten iterations three times (median) with `full`.
+    """
     import numpy as np
+    from gpu_amd.geometry import plain_sart_128
+    from gpu_amd.phantom import phantom
     from oracle import oracle as orc
     threads = orc.num_threads()
+    out = {"unit": "GRays/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "nproc": os.cpu_count()}
+    # SART pair on the first views of the sweep
     v = np.zeros(g.vol_shape, dtype=np.float32)
     t0 = time.perf_counter()
-    for j in range(nviews):
+    for j in range(sweep_views):
         c = orc.sart_correction(g, v, meas_host[j:j + 1], view0=j, nviews=1, prec=32)
         v = orc.backproject_voxel(g, v, c, view0=j, nviews=1, beta=beta, mode=1, prec=32)
     dt = time.perf_counter() - t0
-    rays = nviews * g.nrows * g.ncols
-    return {"value": rays / dt / 1e9, "unit": "GRays/s", "cores": threads, "kind": "port",
-            "sample": f"first {nviews} views of the 512^3 / 1024-view SART sweep (correction + voxel update), "
-                      f"FP32, {dt:.1f} s, OpenMP {threads} threads"}
+    rays = sweep_views * g.nrows * g.ncols
+    out["value"] = rays / dt / 1e9
+    out["sample"] = (f"first {sweep_views} views of the 512^3 / 1024-view SART sweep (correction + voxel update), "
+                     f"FP32, {dt:.1f} s, OpenMP {threads} threads")
+    sweep_s = dt / sweep_views * g.nviews
+    # TV-FGP and FISTA at 512^3
+    nfgp = niter if full else 2
+    t0 = time.perf_counter()
+    orc.tv_fgp(v, lam, nfgp, prec=32)
+    fgp_s = (time.perf_counter() - t0) / nfgp * niter
+    xp = np.zeros_like(v)
+    t0 = time.perf_counter()
+    orc.fista_momentum(v, xp, 0.5, prec=32)
+    fista_s = time.perf_counter() - t0
+    out["sart_fista_tv_512"] = {
+        "sweep_s_extrapolated": round(sweep_s, 1), "tv_fgp_s": round(fgp_s, 2), "fista_s": round(fista_s, 3),
+        "s_per_iter_extrapolated": round(sweep_s + fgp_s + fista_s, 1),
+        "note": f"sweep extrapolated x{g.nviews / sweep_views:.0f} from {sweep_views} views"
+                + ("" if full else f"; TV-FGP({niter}) extrapolated from {nfgp} inner iterations")}
+    # FDK of 4 views
+    g4 = g.with_views(4)
+    t0 = time.perf_counter()
+    orc.fdk(g4, meas_host[:4], "hann", prec=32)
+    out["fdk_512_4_views_s"] = round(time.perf_counter() - t0, 2)
+    # the 128^3 / 180-view SART workload
+    g1 = plain_sart_128()
+    x1 = phantom(g1, seed=0).numpy()
+    b1 = orc.forward(g1, x1, prec=32)
+    iters, runs = (10, 3) if full else (1, 1)
+    times = []
+    for _ in range(runs):
+        r = np.zeros_like(x1)
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            r = orc.sart_sweep(g1, r, b1, beta=1.0, subset=20, order="interleaved", prec=32)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    out["plain_sart_128"] = {"iterations": iters, "runs": runs, "median_s": round(med, 2),
+                             "s_per_iter": round(med / iters, 2),
+                             "grays": round(g1.rays * iters / med / 1e9, 5)}
+    return out
 
 
 def style_figures(ctx, g, x32) -> dict:
@@ -258,12 +321,19 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-views", type=int, default=8)
+    ap.add_argument("--cpu-views", type=int, default=6, help="SART views of the CPU baseline's sweep sample")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline over the whole plan: the 128^3 workload's ten iterations three times "
+                         "(median), TV-FGP(20) at 512^3, 32 sweep views")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 reference iterations (accuracy)")
+    ap.add_argument("--acc-iters", type=int, default=3,
+                    help="SART-FISTA-TV iterations of the accuracy pass (FP64, FP32, FP16, BF16 from zero)")
     ap.add_argument("--accuracy", type=int, default=0,
                     help="accuracy mode: N SART-FISTA-TV iterations in FP64 (the GPU FP64 instantiation as the "
                          "oracle), FP32, FP16 and BF16; prints per-iteration RMSE / relative L2 against FP64 and "
                          "the phantom as one JSON line, then exits")
+    ap.add_argument("--beta", type=float, default=0.0,
+                    help="SART relaxation; 0 (default) takes 2/Lip from 30 power iterations of the SART operator")
     ap.add_argument("--pmc-sample", type=int, default=0,
                     help="counter-collection mode: SART sweeps over N views from 0 and from 90 degrees, an FDK over "
                          "the first N views and two TV-FGP inner iterations at full shape, plus a 1 GiB device "
@@ -282,9 +352,11 @@ def main() -> int:
     from gpu_amd.phantom import phantom, poisson_log
 
     g = repo_scale_512()
-    # relaxation 2/Lip_con = 2/32, the reference's SART-FISTA step
-    # (V63/main_recon.cu:58,161); FGP argument lambda = 1e-3
-    beta, lam, niter = 0.0625, 1e-3, 20
+    # FGP argument lambda = 1e-3, 20 inner iterations; the SART-FISTA step
+    # beta = 2/Lip with Lip from the SART operator's power iteration (it
+    # replaces the reference's constant Lip_con = 32, V63/main_recon.cu:58,161)
+    lam, niter = 1e-3, 20
+    beta = args.beta if args.beta > 0 else 0.0625
     ctx = Context(0)
     if args.pmc_sample:
         # counter-collection mode: a cheap input (one ellipsoid-free ball) so the
@@ -323,8 +395,17 @@ def main() -> int:
     del clean
     torch.cuda.synchronize()
 
+    lip = None
+    if args.beta <= 0:
+        log("SART step: 30 power iterations of the SART operator")
+        t0 = time.perf_counter()
+        lip = ctx.sart_lipschitz(g, torch.float32, niter=30, seed=1)
+        beta = 2.0 / lip
+        lip_s = time.perf_counter() - t0
+        log(f"SART step: Lip {lip:.6f}, beta {beta:.6f} ({lip_s:.1f} s)")
     if args.accuracy:
-        return accuracy(ctx, g, x_true, meas32, args.accuracy, beta, lam, niter)
+        print(json.dumps(accuracy(ctx, g, x_true, meas32, args.accuracy, beta, lam, niter)), flush=True)
+        return 0
 
     log(f"FP32 path: warmup {args.warmup}, steps {args.steps}")
     ms32, prof32, x32 = run_path(ctx, g, meas32, torch.float32, args.steps, args.warmup, beta, lam, niter)
@@ -370,7 +451,9 @@ def main() -> int:
         "data": "synthetic: seeded 10-ellipsoid phantom, Poisson I0=1e5 + log",
         "config": {
             "workload": "SART-FISTA-TV iteration, 512^3 x 0.5 mm, 1024 views x 1024^2 x 0.4 mm, SID/SDD 1000/1500 mm",
-            "sart": "view by view (subset 1), relaxation 0.0625 (2/Lip_con)",
+            "sart": (f"view by view (subset 1), relaxation {beta:.6f} = 2/Lip, Lip = {lip:.6f} from 30 power "
+                     "iterations of the SART operator (FP32 projections, FP64 norms)") if lip else
+                    f"view by view (subset 1), relaxation {beta}",
             "tv_fgp": {"inner_iterations": niter, "lambda": lam},
             "rays_per_sweep": rays_per_sweep,
         },
@@ -425,49 +508,45 @@ def main() -> int:
     log("files: " + ", ".join(f"{k} {v}" for k, v in result["io"].items() if k != "note"))
 
     if not args.no_fp16:
-        log("FP16 path")
-        meas16 = meas32.to(torch.float16)
-        ms16, prof16, x16 = run_path(ctx, g, meas16, torch.float16, args.steps, args.warmup, beta, lam, niter)
-        f16 = prof16["forward"][0] + prof16["voxel_bp"][0]
-        grays16 = rays_per_sweep * args.steps / (f16 * 1e-3) / 1e9
-        diff = (x16.float() - x32).double()
-        rmse = float(torch.sqrt((diff * diff).mean()))
-        result["s_per_iter"]["f16"] = round(ms16 / 1e3, 4)
-        result["fp16"] = {"grays": round(grays16, 4), "speedup_vs_f32": round(ms32 / ms16, 3),
-                          "rmse_vs_f32_per_mm": rmse,
-                          "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof16.items() if v[1]}}
-        log(f"FP16: {ms16 / 1e3:.3f} s/iter, GRays/s {grays16:.3f}, speedup {ms32 / ms16:.3f}, rmse {rmse:.3e}")
-        del meas16
+        # the reduced-precision paths: FP16 and BF16 storage, FP32 accumulation,
+        # the same schedule and step
+        for tag, dt in (("f16", torch.float16), ("bf16", torch.bfloat16)):
+            log(f"{tag.upper()} path")
+            meas_r = meas32.to(dt)
+            ms_r, prof_r, x_r = run_path(ctx, g, meas_r, dt, args.steps, args.warmup, beta, lam, niter)
+            f_r = prof_r["forward"][0] + prof_r["voxel_bp"][0]
+            grays_r = rays_per_sweep * args.steps / (f_r * 1e-3) / 1e9
+            diff = (x_r.float() - x32).double()
+            rmse = float(torch.sqrt((diff * diff).mean()))
+            result["s_per_iter"][tag] = round(ms_r / 1e3, 4)
+            result["fp16" if tag == "f16" else tag] = {
+                "grays": round(grays_r, 4), "speedup_vs_f32": round(ms32 / ms_r, 3),
+                "rmse_vs_f32_per_mm": rmse,
+                "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof_r.items() if v[1]}}
+            log(f"{tag.upper()}: {ms_r / 1e3:.3f} s/iter, GRays/s {grays_r:.3f}, speedup {ms32 / ms_r:.3f}, "
+                f"rmse vs f32 {rmse:.3e}")
+            del meas_r, x_r
+        result["fp16"]["target"] = {"speedup_vs_f32": 1.7, "met": result["fp16"]["speedup_vs_f32"] >= 1.7}
+        torch.cuda.empty_cache()
 
     if not args.no_f64:
-        # error against the FP64 oracle: the FP64 instantiation of the same
-        # kernels, run for the same number of iterations on the same data
-        # (outside the timed region)
-        log(f"FP64 reference: {args.warmup + args.steps} iterations")
-        x64 = torch.zeros(g.vol_shape, device="cuda", dtype=torch.float64)
-        xp64 = torch.zeros_like(x64)
-        t64 = 1.0
-        for _ in range(args.warmup + args.steps):
-            t64 = ctx.sart_fista_iteration(g, x64, xp64, meas32, t64, beta=beta, subset=1, fgp_lambda=lam,
-                                           fgp_niter=niter)
-        torch.cuda.synchronize()
-        ref_n = float(torch.linalg.vector_norm(xp64))
-        acc = {"iterations": args.warmup + args.steps, "reference": "FP64 GPU instantiation of the same kernels"}
-        for k, xk in (("f32", x32), ("f16", x16 if not args.no_fp16 else None)):
-            if xk is None:
-                continue
-            d = xk.double() - xp64
-            acc[k] = {"rmse_per_mm": float(torch.sqrt((d * d).mean())),
-                      "rel_l2": float(torch.linalg.vector_norm(d)) / ref_n}
+        # error against the FP64 oracle: FP64, FP32, FP16 and BF16 runs of the
+        # same kernels from zero on the same data, per iteration (outside the
+        # timed region)
+        log(f"accuracy pass: {args.acc_iters} iterations x FP64/FP32/FP16/BF16")
+        acc = accuracy(ctx, g, x_true, meas32, args.acc_iters, beta, lam, niter)
+        last = acc["iterations"][-1]
+        acc["bar"] = {"rmse_vs_f64_per_mm": 1e-3,
+                      "met": all(last[k]["rmse_vs_f64"] <= 1e-3 for k in ("f16", "bf16"))}
         result["accuracy_vs_f64"] = acc
-        log(f"vs FP64: " + ", ".join(f"{k} rmse {v['rmse_per_mm']:.3e} rel {v['rel_l2']:.3e}"
-                                     for k, v in acc.items() if isinstance(v, dict)))
-        del x64, xp64
+        log("vs FP64 after %d: " % args.acc_iters + ", ".join(
+            f"{k} rmse {last[k]['rmse_vs_f64']:.3e} rel {last[k]['rel_l2_vs_f64']:.3e}" for k in ("f32", "f16", "bf16")))
 
     if not args.no_cpu:
-        log(f"CPU baseline: {args.cpu_views} views")
-        meas_host = meas32[: args.cpu_views].cpu().numpy()
-        result["cpu_baseline"] = cpu_baseline(g, meas_host, args.cpu_views, beta)
+        nsv = 32 if args.cpu_full else args.cpu_views
+        log(f"CPU baseline: {nsv} sweep views{' (full plan)' if args.cpu_full else ''}")
+        meas_host = meas32[: max(nsv, 4)].cpu().numpy()
+        result["cpu_baseline"] = cpu_baseline(g, meas_host, beta, lam, niter, nsv, full=args.cpu_full)
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

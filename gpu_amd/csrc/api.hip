@@ -64,6 +64,13 @@ float view_cos(const ct_geometry &g, int j) {
     return c;
 }
 
+void fill_angle_table(const ct_geometry &g, bool dbl, bool fdk, void *host) {
+    for (int v = 0; v < g.nviews; ++v) {
+        if (dbl) { double s, c; angle_host<double>(g, v, fdk, s, c); ((double *)host)[2 * v] = s; ((double *)host)[2 * v + 1] = c; }
+        else { float s, c; angle_host<float>(g, v, fdk, s, c); ((float *)host)[2 * v] = s; ((float *)host)[2 * v + 1] = c; }
+    }
+}
+
 const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fdk, ct_status *st) {
     std::vector<double> key = {(double)g.nviews, g.angle_step_deg, g.angle0_deg, g.angle_shift_deg,
                                dbl ? 1.0 : 0.0, fdk ? 1.0 : 0.0};
@@ -84,10 +91,7 @@ const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fd
         hipStreamSynchronize(ctx->stream);
     }
     std::vector<char> host(bytes);
-    for (int v = 0; v < g.nviews; ++v) {
-        if (dbl) { double s, c; angle_host<double>(g, v, fdk, s, c); ((double *)host.data())[2 * v] = s; ((double *)host.data())[2 * v + 1] = c; }
-        else { float s, c; angle_host<float>(g, v, fdk, s, c); ((float *)host.data())[2 * v] = s; ((float *)host.data())[2 * v + 1] = c; }
-    }
+    fill_angle_table(g, dbl, fdk, host.data());
     hipError_t e = hipMemcpy(ctx->angles.ptr, host.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) { *st = hip_check(ctx, e, "angle table upload"); return nullptr; }
     ctx->angle_key = key;
@@ -189,10 +193,25 @@ static ct_status sweep(ct_context *ctx, const ct_geometry &g, void *vol, ct_dtyp
         void *volq = slot(ctx, SLOT_ZQVOL, bytes, &st);
         if (!volq) return st;
         if ((st = launch_zq_pack(ctx, vol, vdt, volq, g.nx, g.ny, g.nz)) != CT_OK) return st;
+        // views whose rays run along y take the transposed copy, so that all
+        // of them walk by column; the copy follows the sweep's direction
+        // changes (four per orbit in view order)
+        void *volt = nullptr;
+        bool tr = false;
         auto step = [&](int first, int stride, int cnt) {
-            ct_status s2 = launch_correction_zq(ctx, g, volq, vdt, (float *)corr, meas, mdt, first, stride, cnt);
+            const bool want = subset == 1 || cnt == 1 ? sweep_view_transposed(g, first)
+                                                      : sweep_view_transposed(g, first + stride * (cnt / 2));
+            if (want != tr) {
+                if (!volt && !(volt = slot(ctx, SLOT_ZQVOL_T, bytes, &st))) return st;
+                ct_status s0 = want ? launch_zq_transpose(ctx, volq, vdt, volt, g.nx, g.ny, g.nz, true)
+                                    : launch_zq_transpose(ctx, volt, vdt, volq, g.nx, g.ny, g.nz, false);
+                if (s0 != CT_OK) return s0;
+                tr = want;
+            }
+            void *v = tr ? volt : volq;
+            ct_status s2 = launch_correction_zq(ctx, g, v, vdt, (float *)corr, meas, mdt, first, stride, cnt, tr);
             if (s2 != CT_OK) return s2;
-            return launch_voxel_bp_zq(ctx, g, volq, vdt, corr, first, stride, cnt, beta);
+            return launch_voxel_bp_zq(ctx, g, v, vdt, corr, first, stride, cnt, beta, tr);
         };
         if (order == CT_ORDER_INTERLEAVED && subset > 1) {
             const int nsub = (nv + subset - 1) / subset;
@@ -202,6 +221,7 @@ static ct_status sweep(ct_context *ctx, const ct_geometry &g, void *vol, ct_dtyp
             for (int p0 = 0; p0 < nv; p0 += subset)
                 if ((st = step(p0, 1, std::min(subset, nv - p0))) != CT_OK) return st;
         }
+        if (tr && (st = launch_zq_transpose(ctx, volt, vdt, volq, g.nx, g.ny, g.nz, false)) != CT_OK) return st;
         return launch_zq_unpack(ctx, volq, vdt, vol, g.nx, g.ny, g.nz);
     }
     if (order == CT_ORDER_INTERLEAVED && subset > 1) {
@@ -262,6 +282,9 @@ ct_status ct_create(ct_context **out, int device) {
     ct_context *ctx = new ct_context();
     ctx->device = device;
     ctx->stream = nullptr;
+    // the allocation-free entry points' scratch address on this device
+    const ct_status st = noalloc_init(ctx);
+    if (st != CT_OK) { delete ctx; return st; }
     *out = ctx;
     return CT_OK;
 }
@@ -348,6 +371,17 @@ ct_status ct_forward(ct_context *ctx, const ct_geometry *g, const void *vol, ct_
     CT_TRY(views_ok(ctx, g, view0, nviews));
     DeviceGuard dg(ctx);
     return launch_forward(ctx, *g, vol, vdt, proj, pdt, nullptr, pdt, view0, 1, nviews, 0);
+}
+
+ct_status ct_forward_noalloc(ct_context *ctx, const ct_geometry *g, const void *vol, ct_dtype vdt, void *proj,
+                             ct_dtype pdt, int32_t view0, int32_t nviews) {
+    CT_CTX(ctx);
+    CT_TRY(validate(ctx, g));
+    CT_TRY(dt_ok(ctx, vdt)); CT_TRY(dt_ok(ctx, pdt));
+    CT_TRY(check_ptr(ctx, vol, vdt, "vol")); CT_TRY(check_ptr(ctx, proj, pdt, "proj"));
+    CT_TRY(views_ok(ctx, g, view0, nviews));
+    DeviceGuard dg(ctx);
+    return launch_forward_noalloc(ctx, *g, vol, vdt, proj, pdt, view0, nviews);
 }
 
 ct_status ct_sart_correction(ct_context *ctx, const ct_geometry *g, const void *vol, ct_dtype vdt, const void *meas,
@@ -453,6 +487,57 @@ ct_status ct_power_iteration(ct_context *ctx, const ct_geometry *g, ct_dtype dt,
         CT_TRY(launch_axpby(ctx, w, v, dt, V, 1.0 / lam, 0.0));
     }
     *lip = lam;
+    return CT_OK;
+}
+
+// Largest eigenvalue of the operator a SART update applies,
+// T v = D^-1 B^T M^-1 A v: A the Siddon forward projection of every view, M
+// the rays' lengths, B^T the voxel-driven backprojection and D its per-voxel
+// chord sums (the reference's correction and update,
+// V63/kernel_IterativeRecon_universal.cu:554-563, :785-807, taken over all
+// views as one subset).  Each application is a correction pass against zero
+// measurements, c = (0 - A v)/len, followed by a backprojection of c into a
+// zeroed volume with relaxation -1, which leaves T v.  The operator averages
+// along rays and over the rays through a voxel, so its spectral radius is 1
+// up to the reconstruction mask and thresholds; the estimate replaces the
+// fixed Lip_con of V63/main_recon.cu:58 (the step is 2 / lip with
+// lip = 2 * radius).  The iterate starts from 1 + 0.1 N(0, 1) (its Perron
+// component is then large) and is normalised with FP64 norms.
+ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dt, int32_t niter, uint64_t seed,
+                            double *lip) {
+    CT_CTX(ctx);
+    CT_TRY(validate(ctx, g));
+    if (dt != CT_F32 && dt != CT_F64) return fail(ctx, CT_ERR_DTYPE, "SART power iteration runs in F32 or F64");
+    if (!lip) return fail(ctx, CT_ERR_NULL, "result pointer is NULL");
+    if (niter < 1) return fail(ctx, CT_ERR_INVALID_ARG, "niter must be >= 1");
+    DeviceGuard dg(ctx);
+    const size_t es = dtype_size(dt);
+    const int64_t V = vox(g), P = pix(g) * g->nviews;
+    ct_status st;
+    void *v = slot(ctx, SLOT_TMP_VOL2, es * V, &st); if (!v) return st;
+    void *w = slot(ctx, SLOT_MISC, es * V, &st); if (!w) return st;
+    void *c = slot(ctx, SLOT_TMP_PROJ, es * P, &st); if (!c) return st;
+    void *zero = slot(ctx, SLOT_TMP_PROJ2, es * P, &st); if (!zero) return st;
+    CT_TRY(launch_fill(ctx, zero, dt, P, 0.0));
+    CT_TRY(launch_randn(ctx, w, dt, V, seed));
+    CT_TRY(launch_fill(ctx, v, dt, V, 1.0));
+    CT_TRY(launch_axpby(ctx, w, v, dt, V, 0.1, 1.0));
+    double nrm2 = 0;
+    CT_TRY(reduce_dot(ctx, v, v, V, dt, 0, &nrm2));
+    if (!(nrm2 > 0)) return fail(ctx, CT_ERR_INVALID_ARG, "power iteration start vector is zero");
+    CT_TRY(launch_axpby(ctx, v, v, dt, V, 1.0 / std::sqrt(nrm2), 0.0));
+    double lam = 0;
+    for (int it = 0; it < niter; ++it) {
+        CT_TRY(launch_forward_mv(ctx, *g, v, dt, c, dt, zero, dt, 0, 1, g->nviews, 1, 0));
+        CT_TRY(launch_fill(ctx, w, dt, V, 0.0));
+        CT_TRY(launch_voxel_bp(ctx, *g, w, dt, c, 0, 1, g->nviews, -1.0, 1));
+        double wn2 = 0;
+        CT_TRY(reduce_dot(ctx, w, w, V, dt, 0, &wn2));
+        lam = std::sqrt(wn2);
+        if (!(lam > 0)) break;
+        CT_TRY(launch_axpby(ctx, w, v, dt, V, 1.0 / lam, 0.0));
+    }
+    *lip = 2.0 * lam;
     return CT_OK;
 }
 

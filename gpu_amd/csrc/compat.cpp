@@ -17,6 +17,7 @@ struct Compat {
     bool have_geom = false;
     int status = 0;
     std::string error;
+    double ip1 = 0, ip2 = 0;   // the last check's inner products
     void *dvol = nullptr, *dproj = nullptr, *dvol2 = nullptr;
     size_t vol_bytes = 0, proj_bytes = 0, vol2_bytes = 0;
 };
@@ -91,6 +92,44 @@ bool sync() {
     return ok(ct_synchronize(C().ctx), "synchronize");
 }
 
+// <A0 x, A0 x> and <x, B0 A0 x> for view 0, where B0 is `matched` ? the
+// matched adjoint : the voxel-driven backprojection with command 0 into a
+// zeroed volume (x = W where L >= 1e-6 vx), as the reference's check runs it
+// (V63/host_IterativeRecon_CBCT.c:131-204).  Prints both, like the reference,
+// and keeps them for ct_compat_last_inner_products.
+void check_joint(float *h_volume, bool matched, const char *what) {
+    if (!ready(what)) return;
+    Compat &c = C();
+    bool sv;
+    const int64_t P1 = (int64_t)c.g.ncols * c.g.nrows * c.g.nsources;
+    float *v = dev_of(h_volume, 4 * nvox(), c.dvol, c.vol_bytes, true, &sv);
+    if (!v) return;
+    float *p = (float *)grow(c.dproj, c.proj_bytes, 4 * P1);
+    float *w = (float *)grow(c.dvol2, c.vol2_bytes, 4 * nvox());
+    if (!p || !w) { set_err(CT_ERR_ALLOC, "allocation failed"); return; }
+    if (!ok(ct_forward(c.ctx, &c.g, v, CT_F32, p, CT_F32, 0, 1), "forward")) return;
+    if (matched) {
+        if (!ok(ct_adjoint(c.ctx, &c.g, p, CT_F32, w, CT_F32, 0, 1), "adjoint")) return;
+    } else {
+        if (!sync()) return;
+        if (hipMemset(w, 0, 4 * nvox()) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            hipGetLastError();
+            set_err(CT_ERR_DEVICE, "clear failed");
+            return;
+        }
+        if (!ok(ct_backproject_voxel(c.ctx, &c.g, w, CT_F32, p, 0, 1, 1.0, 0), "voxel backprojection")) return;
+    }
+    double ip1 = 0, ip2 = 0;
+    if (!ok(ct_dot(c.ctx, p, p, P1, CT_F32, &ip1), "dot")) return;
+    if (!ok(ct_dot(c.ctx, v, w, nvox(), CT_F32, &ip2), "dot")) return;
+    c.ip1 = ip1;
+    c.ip2 = ip2;
+    std::printf("The 1st inner product = %f\n", ip1);
+    std::printf("The 2nd inner product = %f\n", ip2);
+    set_err(0, "");
+}
+
+
 } // namespace
 
 extern "C" {
@@ -162,26 +201,22 @@ void Reconstruction_3D_ray_driven_TBCT(float *h_volume, float *h_proj_data, floa
 
 void CheckMatchedJointOperator(float *h_volume) {
     std::lock_guard<std::mutex> l(mtx());
-    if (!ready("CheckMatchedJointOperator")) return;
-    Compat &c = C();
-    bool sv, sp, s2;
-    const int64_t P1 = (int64_t)c.g.ncols * c.g.nrows * c.g.nsources;
-    float *v = dev_of(h_volume, 4 * nvox(), c.dvol, c.vol_bytes, true, &sv);
-    if (!v) return;
-    std::vector<float> dummy(1);
-    float *p = (float *)grow(c.dproj, c.proj_bytes, 4 * P1);
-    float *w = (float *)grow(c.dvol2, c.vol2_bytes, 4 * nvox());
-    (void)sp; (void)s2;
-    if (!p || !w) { set_err(CT_ERR_ALLOC, "allocation failed"); return; }
-    if (!ok(ct_forward(c.ctx, &c.g, v, CT_F32, p, CT_F32, 0, 1), "forward")) return;
-    if (!ok(ct_adjoint(c.ctx, &c.g, p, CT_F32, w, CT_F32, 0, 1), "adjoint")) return;
-    double ip1 = 0, ip2 = 0;
-    if (!ok(ct_dot(c.ctx, p, p, P1, CT_F32, &ip1), "dot")) return;
-    if (!ok(ct_dot(c.ctx, v, w, nvox(), CT_F32, &ip2), "dot")) return;
-    std::printf("The 1st inner product = %f\n", ip1);
-    std::printf("The 2nd inner product = %f\n", ip2);
-    set_err(0, "");
+    check_joint(h_volume, false, "CheckMatchedJointOperator");
 }
+
+void ct_compat_check_matched_adjoint(float *h_volume) {
+    std::lock_guard<std::mutex> l(mtx());
+    check_joint(h_volume, true, "ct_compat_check_matched_adjoint");
+}
+
+int ct_compat_last_inner_products(double *ip1, double *ip2) {
+    if (!ip1 || !ip2) return CT_ERR_NULL;
+    *ip1 = C().ip1;
+    *ip2 = C().ip2;
+    return 0;
+}
+
+
 
 void FGP_denoise_GPUx7_exact(float *h_f_noise, float lambda, int Niter_denoise) {
     std::lock_guard<std::mutex> l(mtx());

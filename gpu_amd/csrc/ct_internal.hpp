@@ -212,6 +212,8 @@ struct ct_context {
     std::vector<hipEvent_t> prof_pool;
     double prof_ms[CT_K_COUNT] = {};
     int64_t prof_n[CT_K_COUNT] = {};
+    // host copy of the angle table of the allocation-free forward projection
+    std::vector<char> noalloc_host;
 };
 
 namespace ct {
@@ -262,7 +264,8 @@ enum Slot {
     SLOT_BPTAB = 10,   // per-view tables of the fast voxel-driven backprojector
     SLOT_ZQVOL = 11,   // the sweep's z-quad-layout volume
     SLOT_FDKV = 12,    // per-view constants of the fast FDK backprojector
-    SLOT_IO = 13       // FP32 staging of file reads and writes
+    SLOT_IO = 13,      // FP32 staging of file reads and writes
+    SLOT_ZQVOL_T = 14  // the sweep's volume in the transposed z-quad layout
 };
 
 ct_status fail(ct_context *ctx, ct_status s, const std::string &msg);
@@ -271,6 +274,12 @@ ct_status launch_check(ct_context *ctx, const char *what);
 void *slot(ct_context *ctx, int which, size_t bytes, ct_status *st);
 // Device angle table (T = float or double), sin at [2v], cos at [2v+1].
 const void *angle_table(ct_context *ctx, const ct_geometry &g, bool dbl, bool fdk, ct_status *st);
+// the same table in host memory (g.nviews (sin, cos) pairs, float or double)
+void fill_angle_table(const ct_geometry &g, bool dbl, bool fdk, void *host);
+// looks up this device's address of the allocation-free entry points' scratch
+ct_status noalloc_init(ct_context *ctx);
+ct_status launch_forward_noalloc(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
+                                 ct_dtype odt, int32_t view0, int32_t nv);
 ct_status check_ptr(ct_context *ctx, const void *p, ct_dtype dt, const char *name);
 ct_status validate_geometry(ct_context *ctx, const ct_geometry *g);
 float view_cos(const ct_geometry &g, int j);   // projector angle of view j (the kernels' table)
@@ -300,11 +309,19 @@ ct_status launch_zq_pack(ct_context *ctx, const void *vol, ct_dtype dt, void *vo
 ct_status launch_zq_unpack(ct_context *ctx, const void *volq, ct_dtype dt, void *vol, int M, int N, int Z);
 // z-quad FP64 accumulator -> [z][y][x] volume of type dt
 ct_status launch_zq_unpack_f64(ct_context *ctx, const double *accq, void *vol, ct_dtype dt, int M, int N, int Z);
+// [z/4][y][x][z%4] <-> [z/4][x][y][z%4] (to_tr: towards the transposed one)
+ct_status launch_zq_transpose(ct_context *ctx, const void *in, ct_dtype dt, void *out, int M, int N, int Z,
+                              bool to_tr);
 bool sweep_zq_eligible(const ct_context *ctx, const ct_geometry &g, ct_dtype vdt, int subset);
+// whether the sweep runs view v's correction and update on the transposed
+// z-quad layout (its rays run along y and the column walk is admitted)
+bool sweep_view_transposed(const ct_geometry &g, int v);
+// tr: the volume is in the transposed z-quad layout
 ct_status launch_correction_zq(ct_context *ctx, const ct_geometry &g, const void *volq, ct_dtype vdt, float *corr,
-                               const void *meas, ct_dtype mdt, int32_t first, int32_t stride, int32_t nv);
+                               const void *meas, ct_dtype mdt, int32_t first, int32_t stride, int32_t nv,
+                               bool tr = false);
 ct_status launch_voxel_bp_zq(ct_context *ctx, const ct_geometry &g, void *volq, ct_dtype vdt, const void *corr,
-                             int32_t first, int32_t stride, int32_t nv, double beta);
+                             int32_t first, int32_t stride, int32_t nv, double beta, bool tr = false);
 ct_status launch_forward_mv(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
                             ct_dtype odt, const void *meas, ct_dtype mdt, int32_t first, int32_t stride,
                             int32_t nv, int cmd, int32_t mv0);

@@ -110,6 +110,47 @@ ct_status launch_zq_unpack(ct_context *ctx, const void *volq, ct_dtype dt, void 
     return zq_copy<false>(ctx, volq, dt, vol, M, N, Z);
 }
 
+// Transpose of the lateral axes of a z-quad-layout volume: every z-quad
+// plane holds an R x C matrix of quads (Q = 16 bytes for FP32, 8 for 16-bit
+// storage), written out as C x R, so [z/4][y][x][z%4] becomes
+// [z/4][x][y][z%4] and back.  32 x 32 quad tiles go through LDS so that both
+// the reads and the writes are row-contiguous.
+template <typename Q>
+__global__ __launch_bounds__(256) void k_zq_transpose(const Q *__restrict__ in, Q *__restrict__ out, int R, int C) {
+    __shared__ Q tile[32][33];
+    const int64_t plane = (int64_t)blockIdx.z * R * C;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int r = r0 + ty + k, c = c0 + tx;
+        if (r < R && c < C) tile[ty + k][tx] = in[plane + (int64_t)r * C + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int c = c0 + ty + k, r = r0 + tx;
+        if (r < R && c < C) out[plane + (int64_t)c * R + r] = tile[tx][ty + k];
+    }
+}
+// to_tr: [z/4][y][x][z%4] -> [z/4][x][y][z%4]; otherwise the reverse
+ct_status launch_zq_transpose(ct_context *ctx, const void *in, ct_dtype dt, void *out, int M, int N, int Z,
+                              bool to_tr) {
+    const int R = to_tr ? N : M, C = to_tr ? M : N;
+    dim3 g((C + 31) / 32, (R + 31) / 32, (Z + 3) / 4), b(256);
+    switch (dt) {
+    case CT_F32:
+        hipLaunchKernelGGL(k_zq_transpose<uint4>, g, b, 0, ctx->stream, (const uint4 *)in, (uint4 *)out, R, C);
+        break;
+    case CT_F16:
+    case CT_BF16:
+        hipLaunchKernelGGL(k_zq_transpose<uint2>, g, b, 0, ctx->stream, (const uint2 *)in, (uint2 *)out, R, C);
+        break;
+    default: return fail(ctx, CT_ERR_DTYPE, "z-quad transpose: unsupported dtype");
+    }
+    return launch_check(ctx, "z-quad transpose");
+}
+
 template <typename SO>
 __global__ __launch_bounds__(256) void k_zq_unpack_f64(const double *__restrict__ in, SO *__restrict__ out, int M, int N,
                                                        int Z) {

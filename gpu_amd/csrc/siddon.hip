@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <mutex>
+#include <string>
 
 namespace ct {
 
@@ -672,7 +674,10 @@ template <> __device__ __forceinline__ float4 load_quad<__hip_bfloat16>(__amdgpu
                        __uint_as_float(y & 0xffff0000u));
 }
 
-template <typename SV, typename SO, typename SM, int CMD>
+// TR: the volume is in the transposed z-quad layout [z/4][x][y][z%4], for
+// views whose rays run along y (the staged quads of consecutive segments then
+// share cache lines, as they do along x in the plain z-quad layout).
+template <typename SV, typename SO, typename SM, int CMD, bool TR = false>
 __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const float *__restrict__ ang,
                                                           const SV *__restrict__ fq, SO *__restrict__ out,
                                                           const SM *__restrict__ meas, int first, int stride,
@@ -736,7 +741,8 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         auto XA = [&](int i) { return fmaf(fmaf((float)i, fsx, px0), X.k, X.b0); };
         auto YA = [&](int j) { return fmaf(fmaf((float)j, fsy, py0), Y.k, Y.b0); };
         auto OFF = [&](int i, int j) {
-            return (unsigned)(((vj0 + j * sgy) * (int)M + (vi0 + i * sgx)) * 4 * ES);
+            if constexpr (TR) return (unsigned)(((vi0 + i * sgx) * d.N + (vj0 + j * sgy)) * 4 * ES);
+            else return (unsigned)(((vj0 + j * sgy) * (int)M + (vi0 + i * sgx)) * 4 * ES);
         };
         // first staged quad of a segment [ap, a]: the z voxel range of the
         // workgroup's first and last rows at both ends, one voxel of margin
@@ -1364,13 +1370,13 @@ static int col_walk_nq(const DGeo<float> &d) {
     return (int)nq;
 }
 
-template <typename SV, typename SO, typename SM, int CMD>
+template <typename SV, typename SO, typename SM, int CMD, bool TR = false>
 static void launch_col(ct_context *ctx, const DGeo<float> &d, int nq, const float *ang, const void *volq, void *out,
                        const void *meas, int first, int stride, int mv0, int nimg) {
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid(d.R, (d.ZP + COL_ROWS - 1) / COL_ROWS, nz);
-        hipLaunchKernelGGL((k_project_col<SV, SO, SM, CMD>), grid, dim3(COL_ROWS), col_dyn_bytes(d.M, d.N),
+        hipLaunchKernelGGL((k_project_col<SV, SO, SM, CMD, TR>), grid, dim3(COL_ROWS), col_dyn_bytes(d.M, d.N),
                            ctx->stream, d, ang, (const SV *)volq, (SO *)out, (const SM *)meas, first, stride, mv0,
                            z0, nq);
     }
@@ -1504,9 +1510,105 @@ ct_status launch_forward_mv(ct_context *ctx, const ct_geometry &g, const void *v
     return fail(ctx, CT_ERR_DTYPE, "unsupported volume dtype");
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free forward projection (ct_forward_noalloc).  Its only scratch
+// is the per-view (sin, cos) table the kernels read, which ct_forward keeps
+// in a context buffer allocated on first use.  Here the table lives in the
+// file-scope array `scratch` (256 KiB: up to 16384 views, in FP64 or FP32),
+// one copy per GPU, held until the process ends.  Its device address
+// is looked up once per device at context creation (noalloc_init) and only
+// that address is passed to kernels and copies.  Calls that overlap on one
+// GPU (two streams or two contexts) race on the table; a call first waits for
+// its own stream, so back-to-back calls on one stream are ordered.
+constexpr int NOALLOC_MAX_VIEWS = 16384;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_VIEWS];
+
+static std::mutex g_noalloc_mtx;
+static void *g_noalloc_ptr[64] = {};
+
+ct_status noalloc_init(ct_context *ctx) {
+    if (ctx->device < 0 || ctx->device >= 64) return fail(ctx, CT_ERR_DEVICE, "device index beyond 63");
+    ctx->noalloc_host.assign(sizeof(scratch), 0);   // the host side of the table, allocated here once
+    std::lock_guard<std::mutex> l(g_noalloc_mtx);
+    if (g_noalloc_ptr[ctx->device]) return CT_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        hipGetLastError();
+        return fail(ctx, CT_ERR_DEVICE, std::string("scratch symbol lookup failed: ") + hipGetErrorString(e));
+    }
+    g_noalloc_ptr[ctx->device] = p;
+    return CT_OK;
+}
+
+// The per-ray forward walk of run_forward (no z-quad copy) on a caller-given
+// angle table.
+template <typename T, typename SV, typename SO>
+static ct_status run_forward_table(ct_context *ctx, const ct_geometry &g, const T *ang, const void *vol, void *out,
+                                   int first, int nv) {
+    DGeo<T> d = derive<T>(g);
+    const int nimg = nv * d.NS;
+    const bool comp = std::is_same<T, float>::value && ctx->math == CT_MATH_FAST;
+    for (int z0 = 0; z0 < nimg; z0 += 65535) {
+        const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
+        dim3 grid((d.R + 15) / 16, (d.ZP + 15) / 16, nz);
+        if (comp)
+            hipLaunchKernelGGL((k_forward<T, SV, SO, float, 0, true>), grid, dim3(256), 0, ctx->stream, d, ang,
+                               (const SV *)vol, (SO *)out, (const float *)nullptr, first, 1, first, z0);
+        else
+            hipLaunchKernelGGL((k_forward<T, SV, SO, float, 0, false>), grid, dim3(256), 0, ctx->stream, d, ang,
+                               (const SV *)vol, (SO *)out, (const float *)nullptr, first, 1, first, z0);
+    }
+    return launch_check(ctx, "forward (allocation-free)");
+}
+
+template <typename T, typename SV>
+static ct_status noalloc_out(ct_context *ctx, const ct_geometry &g, const T *ang, const void *vol, void *out,
+                             ct_dtype odt, int first, int nv) {
+    switch (odt) {
+    case CT_F32: return run_forward_table<T, SV, float>(ctx, g, ang, vol, out, first, nv);
+    case CT_F16: return run_forward_table<T, SV, __half>(ctx, g, ang, vol, out, first, nv);
+    case CT_BF16: return run_forward_table<T, SV, __hip_bfloat16>(ctx, g, ang, vol, out, first, nv);
+    case CT_F64: return run_forward_table<T, SV, double>(ctx, g, ang, vol, out, first, nv);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "unsupported output dtype");
+}
+
+ct_status launch_forward_noalloc(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,
+                                 ct_dtype odt, int32_t view0, int32_t nv) {
+    if (g.nviews > NOALLOC_MAX_VIEWS)
+        return fail(ctx, CT_ERR_INVALID_ARG, "allocation-free forward: nviews " + std::to_string(g.nviews) +
+                                                 " exceeds its limit of " + std::to_string(NOALLOC_MAX_VIEWS));
+    if (nv <= 0) return CT_OK;
+    void *tab = nullptr;
+    {
+        std::lock_guard<std::mutex> l(g_noalloc_mtx);
+        if (ctx->device >= 0 && ctx->device < 64) tab = g_noalloc_ptr[ctx->device];
+    }
+    if (!tab) return fail(ctx, CT_ERR_DEVICE, "allocation-free forward: scratch not set up on this device");
+    // the previous call on this stream may still read the table
+    ct_status st = hip_check(ctx, hipStreamSynchronize(ctx->stream), "stream synchronize");
+    if (st != CT_OK) return st;
+    const bool dbl = vdt == CT_F64;
+    const size_t bytes = (size_t)g.nviews * 2 * (dbl ? 8 : 4);
+    // sized for NOALLOC_MAX_VIEWS at context creation (noalloc_init)
+    if (ctx->noalloc_host.size() < bytes) return fail(ctx, CT_ERR_INVALID_ARG, "allocation-free forward: host table");
+    fill_angle_table(g, dbl, false, ctx->noalloc_host.data());
+    st = hip_check(ctx, hipMemcpyAsync(tab, ctx->noalloc_host.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+                   "angle table upload");
+    if (st != CT_OK) return st;
+    switch (vdt) {
+    case CT_F32: return noalloc_out<float, float>(ctx, g, (const float *)tab, vol, out, odt, view0, nv);
+    case CT_F16: return noalloc_out<float, __half>(ctx, g, (const float *)tab, vol, out, odt, view0, nv);
+    case CT_BF16: return noalloc_out<float, __hip_bfloat16>(ctx, g, (const float *)tab, vol, out, odt, view0, nv);
+    case CT_F64: return noalloc_out<double, double>(ctx, g, (const double *)tab, vol, out, odt, view0, nv);
+    }
+    return fail(ctx, CT_ERR_DTYPE, "unsupported volume dtype");
+}
+
 template <typename SV, typename SM>
 static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const void *volq, float *corr,
-                                   const void *meas, int first, int stride, int nv) {
+                                   const void *meas, int first, int stride, int nv, bool tr) {
     ct_status st;
     const float *ang = (const float *)angle_table(ctx, g, false, false, &st);
     if (!ang) return st;
@@ -1514,6 +1616,13 @@ static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const 
     ProfScope ps(ctx, CT_K_FORWARD);
     const int nq = col_walk_nq(d);
     const int64_t P = (int64_t)d.R * d.ZP;
+    if (tr) {
+        // transposed layout (sweep_view_transposed held for the caller's
+        // views): every view takes the column walk
+        if (nq <= 0) return fail(ctx, CT_ERR_INVALID_ARG, "transposed correction without the column walk");
+        launch_col<SV, float, SM, 1, true>(ctx, d, nq, ang, volq, corr, meas, first, stride, 0, nv * d.NS);
+        return launch_check(ctx, "correction");
+    }
     by_run(g, nq > 0, first, stride, nv, [&](int q0, int n, bool cw) {
         float *o = corr + (int64_t)q0 * d.NS * P;
         const int f0 = first + q0 * stride, ni = n * d.NS;
@@ -1534,12 +1643,12 @@ static ct_status run_correction_zq(ct_context *ctx, const ct_geometry &g, const 
 
 template <typename SV>
 static ct_status corr_zq_meas(ct_context *ctx, const ct_geometry &g, const void *volq, float *corr, const void *meas,
-                              ct_dtype mdt, int first, int stride, int nv) {
+                              ct_dtype mdt, int first, int stride, int nv, bool tr) {
     switch (mdt) {
-    case CT_F32: return run_correction_zq<SV, float>(ctx, g, volq, corr, meas, first, stride, nv);
-    case CT_F16: return run_correction_zq<SV, __half>(ctx, g, volq, corr, meas, first, stride, nv);
-    case CT_BF16: return run_correction_zq<SV, __hip_bfloat16>(ctx, g, volq, corr, meas, first, stride, nv);
-    case CT_F64: return run_correction_zq<SV, double>(ctx, g, volq, corr, meas, first, stride, nv);
+    case CT_F32: return run_correction_zq<SV, float>(ctx, g, volq, corr, meas, first, stride, nv, tr);
+    case CT_F16: return run_correction_zq<SV, __half>(ctx, g, volq, corr, meas, first, stride, nv, tr);
+    case CT_BF16: return run_correction_zq<SV, __hip_bfloat16>(ctx, g, volq, corr, meas, first, stride, nv, tr);
+    case CT_F64: return run_correction_zq<SV, double>(ctx, g, volq, corr, meas, first, stride, nv, tr);
     }
     return fail(ctx, CT_ERR_DTYPE, "unsupported measurement dtype");
 }
@@ -1547,15 +1656,21 @@ static ct_status corr_zq_meas(ct_context *ctx, const ct_geometry &g, const void 
 // The sweep's SART correction on a z-quad-layout volume (views first,
 // first + stride, ...; the measurement stack is indexed by view).
 ct_status launch_correction_zq(ct_context *ctx, const ct_geometry &g, const void *volq, ct_dtype vdt, float *corr,
-                               const void *meas, ct_dtype mdt, int32_t first, int32_t stride, int32_t nv) {
+                               const void *meas, ct_dtype mdt, int32_t first, int32_t stride, int32_t nv, bool tr) {
     if (nv <= 0) return CT_OK;
     switch (vdt) {
-    case CT_F32: return corr_zq_meas<float>(ctx, g, volq, corr, meas, mdt, first, stride, nv);
-    case CT_F16: return corr_zq_meas<__half>(ctx, g, volq, corr, meas, mdt, first, stride, nv);
-    case CT_BF16: return corr_zq_meas<__hip_bfloat16>(ctx, g, volq, corr, meas, mdt, first, stride, nv);
+    case CT_F32: return corr_zq_meas<float>(ctx, g, volq, corr, meas, mdt, first, stride, nv, tr);
+    case CT_F16: return corr_zq_meas<__half>(ctx, g, volq, corr, meas, mdt, first, stride, nv, tr);
+    case CT_BF16: return corr_zq_meas<__hip_bfloat16>(ctx, g, volq, corr, meas, mdt, first, stride, nv, tr);
     default: break;
     }
     return fail(ctx, CT_ERR_DTYPE, "z-quad correction: unsupported volume dtype");
+}
+
+// The sweep walks views whose rays run along y on the transposed z-quad
+// layout, so that every view takes the column walk with line-sharing staging.
+bool sweep_view_transposed(const ct_geometry &g, int v) {
+    return !col_walk_view(g, v) && col_walk_nq(derive<float>(g)) > 0;
 }
 
 ct_status launch_forward(ct_context *ctx, const ct_geometry &g, const void *vol, ct_dtype vdt, void *out,

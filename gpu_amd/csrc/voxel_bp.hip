@@ -475,13 +475,17 @@ __device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__
 // carried across images.  H: the packed-half chord engine (reduced-precision
 // storage paths).  ZQ: the volume is in the sweep's z-quad layout
 // [z/4][y][x][z%4].
-template <bool ZQ>
+// TR: the transposed z-quad layout [z/4][x][y][z%4] of the sweep's
+// y-directed views; the lanes then run along y (threadIdx.x -> y) so that a
+// wave's accesses stay contiguous.
+template <bool ZQ, bool TR = false>
 __device__ __forceinline__ int64_t vox_index(const DGeo<float> &d, int ix, int iy, int iz) {
-    if constexpr (ZQ) return (((int64_t)(iz >> 2) * d.N + iy) * d.M + ix) * 4 + (iz & 3);
+    if constexpr (ZQ && TR) return (((int64_t)(iz >> 2) * d.M + ix) * d.N + iy) * 4 + (iz & 3);
+    else if constexpr (ZQ) return (((int64_t)(iz >> 2) * d.N + iy) * d.M + ix) * 4 + (iz & 3);
     else return ((int64_t)iz * d.N + iy) * d.M + ix;
 }
 
-template <typename SV, bool ONE, bool H, bool ZQ>
+template <typename SV, bool ONE, bool H, bool ZQ, bool TR = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void k_voxel_bp_lds(DGeo<float> d, const float *__restrict__ ang,
                                                       SV *__restrict__ f, const float2 *__restrict__ pixT,
                                                       const float4 *__restrict__ colT,
@@ -493,8 +497,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
     uint2 *const sph = reinterpret_cast<uint2 *>(sp);
     const int tid = threadIdx.y * BPX + threadIdx.x;
     const int3 tile = xcd_tile();   // each XCD: a contiguous z range of tiles
-    const int ix = tile.x * BPX + threadIdx.x;
-    const int iy = tile.y * BPY + threadIdx.y;
+    const int ix = tile.x * BPX + (TR ? threadIdx.y : threadIdx.x);
+    const int iy = tile.y * BPY + (TR ? threadIdx.x : threadIdx.y);
     constexpr int ZR = zrun<ONE>();
     const int iz0 = tile.z * ZR;
     const int nz = min(ZR, d.Z - iz0);
@@ -677,7 +681,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                 }
                 if constexpr (ONE)
-                    bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,
+                    bp_update(d, f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,
                               mode);
                 else { W[k] = wk; L[k] = lk; }
             }
@@ -688,7 +692,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
 #pragma unroll
         for (int k = 0; k < ZR; ++k) {
             if (k >= nz) break;
-            bp_update(d, f, vox_index<ZQ>(d, ix, iy, iz0 + k), H ? W[k] * ikl : W[k], H ? L[k] * ikl : L[k],
+            bp_update(d, f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), H ? W[k] * ikl : W[k], H ? L[k] * ikl : L[k],
                       beta, mode);
         }
     }
@@ -878,7 +882,7 @@ static bool bp_lds_fits(const DGeo<float> &d, int zr) {
 }
 
 // per-view tables, then the LDS-staged update (fast FP32 arithmetic)
-template <typename SV, bool ZQ>
+template <typename SV, bool ZQ, bool TR = false>
 static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *ang, void *vol, const void *corr,
                             int first, int stride, int nv, double beta, int mode) {
     ct_status st;
@@ -900,11 +904,11 @@ static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *
     // reduced-precision storage takes the packed-half chord engine
     constexpr bool H = !std::is_same<SV, float>::value;
     if (nimg == 1)
-        hipLaunchKernelGGL((k_voxel_bp_lds<SV, true, H, ZQ>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
+        hipLaunchKernelGGL((k_voxel_bp_lds<SV, true, H, ZQ, TR>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
                            (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
                            nv, (float)beta, mode);
     else
-        hipLaunchKernelGGL((k_voxel_bp_lds<SV, false, H, ZQ>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
+        hipLaunchKernelGGL((k_voxel_bp_lds<SV, false, H, ZQ, TR>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
                            (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
                            nv, (float)beta, mode);
     return launch_check(ctx, "voxel backprojection");
@@ -962,16 +966,22 @@ bool sweep_zq_eligible(const ct_context *ctx, const ct_geometry &g, ct_dtype vdt
 
 // SART update of a z-quad-layout volume (sweep_zq_eligible holds)
 ct_status launch_voxel_bp_zq(ct_context *ctx, const ct_geometry &g, void *volq, ct_dtype vdt, const void *corr,
-                             int32_t first, int32_t stride, int32_t nv, double beta) {
+                             int32_t first, int32_t stride, int32_t nv, double beta, bool tr) {
     if (nv <= 0) return CT_OK;
     ct_status st;
     const float *ang = (const float *)angle_table(ctx, g, false, false, &st);
     if (!ang) return st;
     const DGeo<float> d = derive<float>(g);
     switch (vdt) {
-    case CT_F32: return run_bp_lds<float, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
-    case CT_F16: return run_bp_lds<__half, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
-    case CT_BF16: return run_bp_lds<__hip_bfloat16, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
+    case CT_F32:
+        return tr ? run_bp_lds<float, true, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1)
+                  : run_bp_lds<float, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
+    case CT_F16:
+        return tr ? run_bp_lds<__half, true, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1)
+                  : run_bp_lds<__half, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
+    case CT_BF16:
+        return tr ? run_bp_lds<__hip_bfloat16, true, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1)
+                  : run_bp_lds<__hip_bfloat16, true>(ctx, d, ang, volq, corr, first, stride, nv, beta, 1);
     default: break;
     }
     return fail(ctx, CT_ERR_DTYPE, "z-quad update: unsupported volume dtype");

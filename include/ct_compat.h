@@ -33,9 +33,16 @@ void        ct_compat_release(void);
 void Forward_3D_ray_driven_siddon(float *h_volume, float *h_proj_data);
 /* V63/host_IterativeRecon_CBCT.c:1 - one SART sweep, view by view, volume updated in place. */
 void Reconstruction_3D_ray_driven_CBCT(float *h_volume, float *h_proj_data, float beta_temp);
-/* V63/host_IterativeRecon_CBCT.c:131 - prints <A0 x, A0 x> and <x, A0^T A0 x> for view 0
- * (A0^T is the matched adjoint here). */
+/* V63/host_IterativeRecon_CBCT.c:131 - prints <A0 x, A0 x> and <x, B0 A0 x> for view 0,
+ * B0 the voxel-driven backprojection with command 0 into a zeroed volume, as
+ * the reference runs it: the gap shows how far that backprojector is from
+ * the transpose. */
 void CheckMatchedJointOperator(float *h_volume);
+/* No counterpart in the reference: the same check with the matched adjoint
+ * (ct_adjoint) as B0, whose two inner products agree to rounding. */
+void ct_compat_check_matched_adjoint(float *h_volume);
+/* The two inner products of the last check (either function above). */
+int  ct_compat_last_inner_products(double *ip1, double *ip2);
 /* V63/host_FGP_Denoise_GPUx7.c:1 - in place. */
 void FGP_denoise_GPUx7_exact(float *h_f_noise, float lambda, int Niter_denoise);
 /* V63/host_FDK.cpp:4 - h_volume += FDK(h_proj_data) (caller zeroes h_volume). */

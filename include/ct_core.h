@@ -156,6 +156,22 @@ ct_status ct_forward(ct_context *ctx, const ct_geometry *g,
                      void *proj, ct_dtype proj_dtype,
                      int32_t view0, int32_t nviews);
 
+/* No counterpart in the reference: ct_forward (CMD 0 ray sums) for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or allocating library routine: its one scratch buffer, the per-view
+ * angle table, is a static 256 KiB device array per GPU, so g->nviews is
+ * limited to 16384 (larger geometries get CT_ERR_INVALID_ARG).  It always
+ * takes the per-ray walk on the volume as given (no z-quad copy); results
+ * agree with ct_forward to FP32 rounding (identical for fewer than four
+ * images).  The call first synchronizes its stream, then overwrites the
+ * table: calls that overlap on one GPU from different streams or contexts
+ * race on it. */
+ct_status ct_forward_noalloc(ct_context *ctx, const ct_geometry *g,
+                             const void *vol, ct_dtype vol_dtype,
+                             void *proj, ct_dtype proj_dtype,
+                             int32_t view0, int32_t nviews);
+
 /* SART correction (command 1, same kernel): corr = (b - sum l*f)/sum l, 0 where
  * sum l < 1e-6*vz.  meas holds the views' measured projections; corr is F32
  * (F64 when vol_dtype is F64). */
@@ -204,6 +220,17 @@ ct_status ct_adjoint_test(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
 /* Largest eigenvalue of A^T A by niter power iterations (FP64 norms). */
 ct_status ct_power_iteration(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
                              int32_t niter, uint64_t seed, double *lipschitz);
+
+/* Lipschitz estimate of the SART step: 2 x the largest eigenvalue of the
+ * operator one SART update applies over all views, T = D^-1 B^T M^-1 A (A the
+ * forward projection, M the rays' lengths, B^T the voxel-driven
+ * backprojection, D its per-voxel chord sums), by niter power iterations with
+ * FP64 norms; dtype is the arithmetic of the projections (F32 or F64).  The
+ * SART-FISTA step is beta = 2 / lipschitz and the FGP argument
+ * 2 lambda_TV / lipschitz, replacing the constant Lip_con of
+ * V63/main_recon.cu:58,161-162. */
+ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
+                            int32_t niter, uint64_t seed, double *lipschitz);
 
 /* ---- FISTA / TV ------------------------------------------------------- */
 /* y = x + gamma*(x - x_prev); x_prev = x (V63/main_recon.cu:375-382).  y may

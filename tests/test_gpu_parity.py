@@ -625,14 +625,16 @@ def test_reduced_precision_storage(ctx, dt):
 
 
 @pytest.mark.parametrize("dt,tol", [(torch.float16, 3e-3), (torch.bfloat16, 1.2e-2)])
-@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fan"])
+@pytest.mark.parametrize("name", ["small", "offsets", "aniso", "multisource", "fan", "fine"])
 @pytest.mark.parametrize("mode,nv", [(0, 1), (1, 1), (1, 3)])
 def test_backproject_reduced_precision(ctx, dt, tol, name, mode, nv):
     """FP16/BF16 volumes take the packed-half chord engine: chord intervals in
     voxel-local FP16 units, products accumulated in FP32, and the updated
     value stored with stochastic rounding.  Against the FP32 oracle on the
     storage-rounded input, the error is that of the stored result (up to one
-    ulp: FP16 1e-3, BF16 8e-3 relative) plus the engine's ~1e-3 on W/L."""
+    ulp: FP16 1e-3, BF16 8e-3 relative) plus the engine's ~1e-3 on W/L.
+    "fine" takes the per-voxel kernel (footprints too wide for the LDS
+    window), whose 16-bit updates are stochastically rounded too."""
     g = GEOMS[name]
     rng = np.random.default_rng(8)
     x = torch.as_tensor(phantom(g, seed=18, scale=0.3).numpy()).to(dt)
@@ -645,6 +647,39 @@ def test_backproject_reduced_precision(ctx, dt, tol, name, mode, nv):
     r = rel(got, ref)
     print(f"{name} {dt} mode {mode} nv {nv}: rel {r:.3e}")
     assert r <= tol
+
+
+@pytest.mark.parametrize("name", ["small", "fine"])
+@pytest.mark.parametrize("x0", [3e-6, 0.01])
+def test_stochastic_rounding_is_unbiased(ctx, name, x0):
+    """64 single-view FP16 updates of 0.39 storage ulp each (constant
+    correction, so every hit voxel gains beta*kappa per view): with
+    stochastic rounding the mean gain over the hit voxels is the exact one to
+    within 5%, in the FP16 subnormal range (x0 = 3e-6, ulp 2^-24) and in the
+    normal range (x0 = 0.01, ulp 2^-17); rounding to nearest would keep
+    none of it.  The LDS-staged kernel ("small") and the per-voxel kernel
+    ("fine") both store this way."""
+    g = GEOMS[name]
+    ulp = 2.0 ** -24 if x0 < 6.1e-5 else 2.0 ** -17
+    kappa = 0.37
+    beta = 0.39 * ulp / kappa
+    v = torch.full(g.vol_shape, x0, device="cuda", dtype=torch.float16)
+    start = host(v)
+    corr = torch.full((1, *g.proj_shape[1:]), kappa, device="cuda")
+    nup = 64
+    for j in range(nup):
+        ctx.backproject_voxel(g, v, corr, view0=j % g.nviews, nviews=1, beta=beta, mode=1)
+    got = host(v) - start
+    # the exact gain: beta*kappa for every view whose update reaches the voxel
+    ones = np.ones((1, *g.proj_shape[1:]))
+    nhit = sum((orc.backproject_voxel(g, np.zeros(g.vol_shape), ones, view0=j % g.nviews, nviews=1, beta=1.0,
+                                      mode=1, prec=64) > 0).astype(np.int64) for j in range(nup))
+    hit = nhit > 0
+    assert hit.sum() > 100
+    want = float((nhit[hit] * beta * kappa).mean())
+    mean = float(got[hit].mean())
+    print(f"{name} x0 {x0}: mean gain {mean:.4e}, exact {want:.4e} ({mean / want:.4f})")
+    assert abs(mean / want - 1) < 0.05
 
 
 def test_compat_layer(ctx):
