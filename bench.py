@@ -108,6 +108,53 @@ def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
     return out
 
 
+def adam_multisource(ctx, iters: int, storage: str) -> dict:
+    """The multi-source gradient workload at its own size: 256^3 at 0.75 mm,
+    48 point sources 3 mm apart on the axis, 720 views of 1024 x 128 at
+    0.5 mm (34560 images, 4.5e9 rays); Adam (lr 1e-3) on
+    0.5 ||A x - b||^2 + 1e-3 TV_eps(x), eps 1e-8, through the autograd
+    projector (forward = ray sums, backward = the matched adjoint, the rays
+    re-walked instead of storing per-ray intermediates), volume stored in
+    `storage` with FP32 master weights, from zero.  b = A x_true of the seeded
+    phantom (FP32).  Reports seconds per iteration (HIP events), the loss per
+    iteration and the distance to the phantom."""
+    import torch
+    from gpu_amd.autograd import project, tv_eps
+    from gpu_amd.geometry import multisource_256
+    from gpu_amd.phantom import phantom
+    g = multisource_256()
+    sdt = {"bf16": torch.bfloat16, "f16": torch.float16, "f32": torch.float32}[storage]
+    x_true = phantom(g, seed=0, scale=2.0, device="cuda")
+    b = ctx.forward(g, x_true)
+    x = torch.zeros(g.vol_shape, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([x], lr=1e-3)
+    lam, eps = 1e-3, 1e-8
+    losses, times = [], []
+    for k in range(iters):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        opt.zero_grad()
+        xs = x.to(sdt)
+        r = project(xs, g, ctx, out_dtype=torch.float32) - b
+        loss = 0.5 * torch.linalg.vector_norm(r, dtype=torch.float64) ** 2 + lam * tv_eps(xs, eps, ctx)
+        loss.backward()
+        opt.step()
+        e1.record()
+        torch.cuda.synchronize()
+        losses.append(float(loss))
+        times.append(e0.elapsed_time(e1) / 1e3)
+        log(f"adam {storage}: iteration {k + 1} loss {losses[-1]:.6e} {times[-1]:.2f} s")
+        del r, loss, xs
+    xt = x_true.double()
+    d = x.detach().double() - xt
+    return {"workload": "multi-source Adam, 256^3 x 0.75 mm, 48 sources x 720 views of 1024 x 128, lr 1e-3, "
+                        "0.5||Ax-b||^2 + 1e-3 TV_eps",
+            "storage": storage, "iterations": iters,
+            "s_per_iteration": sum(times[1:]) / max(len(times) - 1, 1), "first_iteration_s": times[0],
+            "loss": losses, "rmse_vs_phantom": float(torch.sqrt((d * d).mean())),
+            "rel_l2_vs_phantom": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(xt))}
+
+
 def cpu_model() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -332,8 +379,13 @@ def main() -> int:
                     help="accuracy mode: N SART-FISTA-TV iterations in FP64 (the GPU FP64 instantiation as the "
                          "oracle), FP32, FP16 and BF16; prints per-iteration RMSE / relative L2 against FP64 and "
                          "the phantom as one JSON line, then exits")
+    ap.add_argument("--adam", type=int, default=0,
+                    help="multi-source gradient mode: N Adam iterations of the 256^3 / 48-source / 720-view "
+                         "workload (see --adam-storage); prints one JSON line, then exits")
+    ap.add_argument("--adam-storage", choices=("bf16", "f16", "f32"), default="bf16")
     ap.add_argument("--beta", type=float, default=0.0,
-                    help="SART relaxation; 0 (default) takes 2/Lip from 30 power iterations of the SART operator")
+                    help="SART relaxation; 0 (default) takes 2/Lip from 30 power iterations of the operator one "
+                         "view-by-view sweep applies")
     ap.add_argument("--pmc-sample", type=int, default=0,
                     help="counter-collection mode: SART sweeps over N views from 0 and from 90 degrees, an FDK over "
                          "the first N views and two TV-FGP inner iterations at full shape, plus a 1 GiB device "
@@ -353,11 +405,15 @@ def main() -> int:
 
     g = repo_scale_512()
     # FGP argument lambda = 1e-3, 20 inner iterations; the SART-FISTA step
-    # beta = 2/Lip with Lip from the SART operator's power iteration (it
-    # replaces the reference's constant Lip_con = 32, V63/main_recon.cu:58,161)
+    # beta = 2/Lip with Lip from the power iteration of the operator that one
+    # view-by-view sweep applies (it replaces the reference's constant
+    # Lip_con = 32, V63/main_recon.cu:58,161)
     lam, niter = 1e-3, 20
     beta = args.beta if args.beta > 0 else 0.0625
     ctx = Context(0)
+    if args.adam:
+        print(json.dumps(adam_multisource(ctx, args.adam, args.adam_storage)), flush=True)
+        return 0
     if args.pmc_sample:
         # counter-collection mode: a cheap input (one ellipsoid-free ball) so the
         # profiled run has few dispatches besides the kernels of interest
@@ -397,9 +453,9 @@ def main() -> int:
 
     lip = None
     if args.beta <= 0:
-        log("SART step: 30 power iterations of the SART operator")
+        log("SART step: 30 power iterations of the view-by-view sweep operator")
         t0 = time.perf_counter()
-        lip = ctx.sart_lipschitz(g, torch.float32, niter=30, seed=1)
+        lip = ctx.sart_lipschitz(g, torch.float32, niter=30, seed=1, subset=1)
         beta = 2.0 / lip
         lip_s = time.perf_counter() - t0
         log(f"SART step: Lip {lip:.6f}, beta {beta:.6f} ({lip_s:.1f} s)")
@@ -451,8 +507,9 @@ def main() -> int:
         "data": "synthetic: seeded 10-ellipsoid phantom, Poisson I0=1e5 + log",
         "config": {
             "workload": "SART-FISTA-TV iteration, 512^3 x 0.5 mm, 1024 views x 1024^2 x 0.4 mm, SID/SDD 1000/1500 mm",
-            "sart": (f"view by view (subset 1), relaxation {beta:.6f} = 2/Lip, Lip = {lip:.6f} from 30 power "
-                     "iterations of the SART operator (FP32 projections, FP64 norms)") if lip else
+            "sart": (f"view by view (subset 1), relaxation {beta:.6e} = 2/Lip, Lip = {lip:.6f} from 30 power "
+                     "iterations of the sweep operator sum_v D_v^-1 B_v^T M_v^-1 A_v (FP32 projections, FP64 "
+                     "norms)") if lip else
                     f"view by view (subset 1), relaxation {beta}",
             "tv_fgp": {"inner_iterations": niter, "lambda": lam},
             "rays_per_sweep": rays_per_sweep,

@@ -60,7 +60,7 @@ def _load() -> ctypes.CDLL:
         "ct_adjoint": ([P, G, P, I, P, I, I, I], I),
         "ct_adjoint_test": ([P, G, I, U64, DP, DP], I),
         "ct_power_iteration": ([P, G, I, I, U64, DP], I),
-        "ct_sart_lipschitz": ([P, G, I, I, U64, DP], I),
+        "ct_sart_lipschitz": ([P, G, I, I, I, U64, DP], I),
         "ct_fista_momentum": ([P, P, P, P, I64, I, D], I),
         "ct_fista_gamma": ([D, DP, DP], I),
         "ct_tv_fgp": ([P, I, I, I, P, I, D, I], I),
@@ -261,13 +261,16 @@ class Context:
         self._check(_lib.ct_power_iteration(self._h, ctypes.byref(gc), _DT[dtype], niter, seed, ctypes.byref(lip)))
         return lip.value
 
-    def sart_lipschitz(self, g: Geometry, dtype: torch.dtype = torch.float32, niter: int = 30, seed: int = 0) -> float:
-        """2 x the largest eigenvalue of the SART update operator over all views
+    def sart_lipschitz(self, g: Geometry, dtype: torch.dtype = torch.float32, niter: int = 30, seed: int = 0,
+                       subset: int = 1) -> float:
+        """2 x the largest eigenvalue of the operator one SART sweep with
+        subsets of `subset` views applies to first order in its step
         (ct_sart_lipschitz); the SART-FISTA step is 2 / lipschitz."""
         lip = ctypes.c_double()
         self._stream()
         gc = g.to_c()
-        self._check(_lib.ct_sart_lipschitz(self._h, ctypes.byref(gc), _DT[dtype], niter, seed, ctypes.byref(lip)))
+        self._check(_lib.ct_sart_lipschitz(self._h, ctypes.byref(gc), _DT[dtype], subset, niter, seed,
+                                           ctypes.byref(lip)))
         return lip.value
 
     # ---- FISTA / TV

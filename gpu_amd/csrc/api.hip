@@ -503,22 +503,24 @@ ct_status ct_power_iteration(ct_context *ctx, const ct_geometry *g, ct_dtype dt,
 // fixed Lip_con of V63/main_recon.cu:58 (the step is 2 / lip with
 // lip = 2 * radius).  The iterate starts from 1 + 0.1 N(0, 1) (its Perron
 // component is then large) and is normalised with FP64 norms.
-ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dt, int32_t niter, uint64_t seed,
-                            double *lip) {
+ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dt, int32_t subset, int32_t niter,
+                            uint64_t seed, double *lip) {
     CT_CTX(ctx);
     CT_TRY(validate(ctx, g));
     if (dt != CT_F32 && dt != CT_F64) return fail(ctx, CT_ERR_DTYPE, "SART power iteration runs in F32 or F64");
     if (!lip) return fail(ctx, CT_ERR_NULL, "result pointer is NULL");
     if (niter < 1) return fail(ctx, CT_ERR_INVALID_ARG, "niter must be >= 1");
+    if (subset < 1 || subset > g->nviews) return fail(ctx, CT_ERR_INVALID_ARG, "subset must be in [1, nviews]");
     DeviceGuard dg(ctx);
     const size_t es = dtype_size(dt);
-    const int64_t V = vox(g), P = pix(g) * g->nviews;
+    const int64_t V = vox(g), P = pix(g) * subset;
     ct_status st;
     void *v = slot(ctx, SLOT_TMP_VOL2, es * V, &st); if (!v) return st;
     void *w = slot(ctx, SLOT_MISC, es * V, &st); if (!w) return st;
     void *c = slot(ctx, SLOT_TMP_PROJ, es * P, &st); if (!c) return st;
     void *zero = slot(ctx, SLOT_TMP_PROJ2, es * P, &st); if (!zero) return st;
     CT_TRY(launch_fill(ctx, zero, dt, P, 0.0));
+    // start: 1 + 0.1 N(0, 1), near the operator's non-negative Perron vector
     CT_TRY(launch_randn(ctx, w, dt, V, seed));
     CT_TRY(launch_fill(ctx, v, dt, V, 1.0));
     CT_TRY(launch_axpby(ctx, w, v, dt, V, 0.1, 1.0));
@@ -528,9 +530,15 @@ ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dt, 
     CT_TRY(launch_axpby(ctx, v, v, dt, V, 1.0 / std::sqrt(nrm2), 0.0));
     double lam = 0;
     for (int it = 0; it < niter; ++it) {
-        CT_TRY(launch_forward_mv(ctx, *g, v, dt, c, dt, zero, dt, 0, 1, g->nviews, 1, 0));
+        // w = K v: per subset the correction of zero measurements,
+        // c = -M_S^-1 A_S v, then w -= D_S^-1 B_S^T c (mode 1, step -1) into
+        // an accumulator that every subset reads from the same v
         CT_TRY(launch_fill(ctx, w, dt, V, 0.0));
-        CT_TRY(launch_voxel_bp(ctx, *g, w, dt, c, 0, 1, g->nviews, -1.0, 1));
+        for (int p0 = 0; p0 < g->nviews; p0 += subset) {
+            const int cnt = std::min(subset, g->nviews - p0);
+            CT_TRY(launch_forward_mv(ctx, *g, v, dt, c, dt, zero, dt, p0, 1, cnt, 1, p0));
+            CT_TRY(launch_voxel_bp(ctx, *g, w, dt, c, p0, 1, cnt, -1.0, 1));
+        }
         double wn2 = 0;
         CT_TRY(reduce_dot(ctx, w, w, V, dt, 0, &wn2));
         lam = std::sqrt(wn2);

@@ -415,17 +415,29 @@ ct_status launch_fista(ct_context *ctx, void *x, void *xp, void *y, int64_t n, c
 // the arithmetic type: r[v] is the x-face between v and v+e_x, zero on the last
 // x plane (the reference's staggered d_r[i+1], V63/host_FGP_Denoise_GPUx7.c:8).
 // L = b - lambda * D^T(r,s,t)   (FGP_operator_L_Pc, V63/kernel_FGP_Denoise_GPUx7.cu:2-24)
-template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_fgp_L(const S *b, const T *r, const T *s, const T *t, S *Lout, T *Lt,
+// Dual fields stored as D (the arithmetic type T, or the 16-bit storage type
+// of a 16-bit volume: the duals lie in the unit ball, so FP16/BF16 storage
+// moves the result by ~lambda * 1e-3 at most); arithmetic stays in T.
+template <typename T, typename D> __device__ __forceinline__ T ldd(const D *p, int64_t i) {
+    if constexpr (std::is_same<T, D>::value) return p[i];
+    else return (T)ldf(p, i);
+}
+template <typename T, typename D> __device__ __forceinline__ void std_(D *p, int64_t i, T v) {
+    if constexpr (std::is_same<T, D>::value) p[i] = v;
+    else St<D>::st(p, i, (float)v);
+}
+
+template <typename S, typename T, typename D = T>
+__global__ __launch_bounds__(256) void k_fgp_L(const S *b, const D *r, const D *s, const D *t, S *Lout, T *Lt,
                                                int M, int N, int Z, T lambda) {
     const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= M) return;
     const int64_t MN = (int64_t)M * N;
     const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
-    const T rm = i > 0 ? r[v - 1] : (T)0;
-    const T sm = j > 0 ? s[v - M] : (T)0;
-    const T tm = k > 0 ? t[v - MN] : (T)0;
-    const T tl = (r[v] - rm) + (s[v] - sm) + (t[v] - tm);
+    const T rm = i > 0 ? ldd<T>(r, v - 1) : (T)0;
+    const T sm = j > 0 ? ldd<T>(s, v - M) : (T)0;
+    const T tm = k > 0 ? ldd<T>(t, v - MN) : (T)0;
+    const T tl = (ldd<T>(r, v) - rm) + (ldd<T>(s, v) - sm) + (ldd<T>(t, v) - tm);
     T bv;
     if constexpr (std::is_same<S, double>::value) bv = b[v]; else bv = ldf(b, v);
     const T L = bv - lambda * tl;
@@ -451,11 +463,11 @@ constexpr int FEX = FGX + 2, FEY = FGY + 2;  // staged extent: one voxel of bord
 constexpr int FLX = FGX + 1, FLY = FGY + 1;  // L extent: tile + right column + bottom row
 constexpr int FGZ = 64;                      // slices per workgroup
 
-template <typename S, typename T>
-__global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b, const T *__restrict__ rA,
-                                                        const T *__restrict__ sA, const T *__restrict__ tA,
-                                                        T *__restrict__ rB, T *__restrict__ sB, T *__restrict__ tB,
-                                                        T *__restrict__ o, T *__restrict__ p, T *__restrict__ q, int M,
+template <typename S, typename T, typename D = T>
+__global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b, const D *__restrict__ rA,
+                                                        const D *__restrict__ sA, const D *__restrict__ tA,
+                                                        D *__restrict__ rB, D *__restrict__ sB, D *__restrict__ tB,
+                                                        D *__restrict__ o, D *__restrict__ p, D *__restrict__ q, int M,
                                                         int N, int Z, T lambda, T g, T a12) {
     // staged old duals: three slices in flight (written one step ahead of
     // their first reader, so no barrier guards the write); L: two slices
@@ -497,7 +509,7 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
             gr[u] = 0; gs[u] = 0; gt[u] = 0;
             if (soff[u] >= 0) {
                 const int64_t v = (int64_t)k * MN + soff[u];
-                gr[u] = rA[v]; gs[u] = sA[v]; gt[u] = tA[v];
+                gr[u] = ldd<T>(rA, v); gs[u] = ldd<T>(sA, v); gt[u] = ldd<T>(tA, v);
             }
         }
     };
@@ -546,7 +558,7 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
     auto fetch_own = [&](int k) {
         if (own) {
             const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
-            po = o[v]; pp = p[v]; pq = q[v];
+            po = ldd<T>(o, v); pp = ldd<T>(p, v); pq = ldd<T>(q, v);
         }
     };
     // prologue: slices k0 - 1 and k0 staged, L(k0) formed, slice k0 + 1's
@@ -581,27 +593,27 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
         if (k < Z - 1) tt = tt + a12 * (Lv - sL[(k + 1) & 1][l]);
         const T nrm = sqrt(tr * tr + ts * ts + tt * tt);
         const T den = nrm > (T)1 ? nrm : (T)1;
-        if (i < M - 1) { const T on = tr / den; rB[v] = on + g * (on - co); o[v] = on; }
-        else rB[v] = r0;
-        if (j < N - 1) { const T pn = ts / den; sB[v] = pn + g * (pn - cp); p[v] = pn; }
-        else sB[v] = s0;
-        if (k < Z - 1) { const T qn = tt / den; tB[v] = qn + g * (qn - cq); q[v] = qn; }
-        else tB[v] = t0;
+        if (i < M - 1) { const T on = tr / den; std_(rB, v, on + g * (on - co)); std_(o, v, on); }
+        else std_(rB, v, r0);
+        if (j < N - 1) { const T pn = ts / den; std_(sB, v, pn + g * (pn - cp)); std_(p, v, pn); }
+        else std_(sB, v, s0);
+        if (k < Z - 1) { const T qn = tt / den; std_(tB, v, qn + g * (qn - cq)); std_(q, v, qn); }
+        else std_(tB, v, t0);
     }
 }
 
-template <typename S, typename T>
+template <typename S, typename T, typename D = T>
 static ct_status run_fgp(ct_context *ctx, int M, int N, int Z, void *vol, double lambda, int niter) {
     const int64_t V = (int64_t)M * N * Z;
     ct_status st;
     // (the step keeps in-slice offsets in 32 bits)
     if ((int64_t)M * N >= (1ll << 31)) return fail(ctx, CT_ERR_OVERFLOW, "fgp: slice of 2^31 voxels or more");
     // two sets of duals (read one, write the other) and the momentum copies
-    T *w = (T *)slot(ctx, SLOT_FGP, sizeof(T) * V * 9, &st);
+    D *w = (D *)slot(ctx, SLOT_FGP, sizeof(D) * V * 9, &st);
     if (!w) return st;
-    T *dual[2][3] = {{w, w + V, w + 2 * V}, {w + 6 * V, w + 7 * V, w + 8 * V}};
-    T *o = w + 3 * V, *p = w + 4 * V, *q = w + 5 * V;
-    st = launch_fill(ctx, w, sizeof(T) == 8 ? CT_F64 : CT_F32, V * 9, 0.0);
+    D *dual[2][3] = {{w, w + V, w + 2 * V}, {w + 6 * V, w + 7 * V, w + 8 * V}};
+    D *o = w + 3 * V, *p = w + 4 * V, *q = w + 5 * V;
+    st = launch_fill(ctx, w, dtype_of<D>(), V * 9, 0.0);
     if (st != CT_OK) return st;
     dim3 b(256), gr((M + 255) / 256, N, Z);
     dim3 fb(FGX, FGY), fg((M + FGX - 1) / FGX, (N + FGY - 1) / FGY, (Z + FGZ - 1) / FGZ);
@@ -612,20 +624,22 @@ static ct_status run_fgp(ct_context *ctx, int M, int N, int Z, void *vol, double
         const T tk = ((T)1 + std::sqrt((T)1 + (T)4 * tk1 * tk1)) / (T)2;
         const T g = (tk1 - (T)1) / tk;
         ProfScope ps(ctx, CT_K_FGP);
-        T *const *A = dual[it & 1], *const *B = dual[(it + 1) & 1];
-        hipLaunchKernelGGL((k_fgp_step<S, T>), fg, fb, 0, ctx->stream, (const S *)vol, (const T *)A[0],
-                           (const T *)A[1], (const T *)A[2], B[0], B[1], B[2], o, p, q, M, N, Z, lam, g, a12);
+        D *const *A = dual[it & 1], *const *B = dual[(it + 1) & 1];
+        hipLaunchKernelGGL((k_fgp_step<S, T, D>), fg, fb, 0, ctx->stream, (const S *)vol, (const D *)A[0],
+                           (const D *)A[1], (const D *)A[2], B[0], B[1], B[2], o, p, q, M, N, Z, lam, g, a12);
         tk1 = tk;
     }
-    hipLaunchKernelGGL((k_fgp_L<S, T>), gr, b, 0, ctx->stream, (const S *)vol, o, p, q, (S *)vol, (T *)nullptr, M, N, Z, lam);
+    hipLaunchKernelGGL((k_fgp_L<S, T, D>), gr, b, 0, ctx->stream, (const S *)vol, (const D *)o, (const D *)p,
+                       (const D *)q, (S *)vol, (T *)nullptr, M, N, Z, lam);
     return launch_check(ctx, "fgp");
 }
 
 ct_status launch_fgp(ct_context *ctx, int M, int N, int Z, void *vol, ct_dtype dt, double lambda, int niter) {
     switch (dt) {
     case CT_F32: return run_fgp<float, float>(ctx, M, N, Z, vol, lambda, niter);
-    case CT_F16: return run_fgp<__half, float>(ctx, M, N, Z, vol, lambda, niter);
-    case CT_BF16: return run_fgp<__hip_bfloat16, float>(ctx, M, N, Z, vol, lambda, niter);
+    // 16-bit volumes keep their dual fields in the same 16-bit type
+    case CT_F16: return run_fgp<__half, float, __half>(ctx, M, N, Z, vol, lambda, niter);
+    case CT_BF16: return run_fgp<__hip_bfloat16, float, __hip_bfloat16>(ctx, M, N, Z, vol, lambda, niter);
     case CT_F64: return run_fgp<double, double>(ctx, M, N, Z, vol, lambda, niter);
     }
     return fail(ctx, CT_ERR_DTYPE, "fgp: bad dtype");

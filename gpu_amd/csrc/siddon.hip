@@ -460,13 +460,13 @@ constexpr int FAST_B = 16;  // steps per batch: their gathers are in flight toge
 // range-checked buffer loads on 32-bit byte offsets; larger ones use 64-bit
 // addresses with an explicit range test.
 template <typename SV, bool BUF, bool ZQ>
-__device__ __forceinline__ double walk_sum(FastWalk<BUF ? (int)sizeof(SV) : 1, ZQ> &w, const DGeo<float> &d,
-                                           const SV *__restrict__ f) {
+__device__ __forceinline__ float walk_sum(FastWalk<BUF ? (int)sizeof(SV) : 1, ZQ> &w, const DGeo<float> &d,
+                                          const SV *__restrict__ f) {
     // elements of the array (the z-quad layout pads z to a multiple of 4)
     const unsigned nvox = (unsigned)d.M * (unsigned)d.N * (ZQ ? (unsigned)((d.Z + 3) & ~3) : (unsigned)d.Z);
     // (for the 64-bit path the resource is built but never used)
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(f, BUF ? nvox * (unsigned)sizeof(SV) : 0u);
-    double sum = 0.0;
+    float sum = 0.0f, comp = 0.0f;   // Kahan-compensated total of the batch sums
     int guard = d.M + d.N + d.Z + 4;
     do {
         unsigned ids[FAST_B];
@@ -483,9 +483,11 @@ __device__ __forceinline__ double walk_sum(FastWalk<BUF ? (int)sizeof(SV) : 1, Z
         float ps = dl[0] * fv[0];
 #pragma unroll
         for (int u = 1; u < FAST_B; ++u) ps = fmaf(dl[u], fv[u], ps);
-        sum += (double)ps;
+        const float y = ps - comp, t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
     } while (!w.ended() && guard > 0);
-    return sum;
+    return sum - comp;
 }
 
 template <typename SV, bool BUF, bool ZQ = false>
@@ -495,7 +497,7 @@ __device__ __forceinline__ float siddon_walk_fast(const DGeo<float> &d, float s,
     len_out = 0.0f;
     if (!w.init(d, s, c, col, row, src)) return 0.0f;
     len_out = w.len;
-    return (float)(walk_sum<SV, BUF, ZQ>(w, d, f) * (double)w.dist);
+    return walk_sum<SV, BUF, ZQ>(w, d, f) * w.dist;
 }
 
 template <typename SV>
@@ -570,7 +572,7 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
                                                     int z0) {
     constexpr int NR = 256 / S, WPP = 4 / S;   // rays and waves per part
     constexpr int TW = WPP >= 2 ? 16 : 8, TH = WPP == 4 ? 16 : 8;
-    __shared__ double part[S > 1 ? 256 : 1];
+    __shared__ float part[S > 1 ? 256 : 1];
     const int tid = threadIdx.x, p = tid / NR, rr = tid % NR;
     const int w = rr >> 6, l = rr & 63;
     const int3 t = xcd_tile();
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
     const bool inb = col < d.R && row < d.ZP;
     const int q = img / d.NS, src = img - q * d.NS;
     const int v = first + q * stride;
-    double sum = 0.0;
+    float sum = 0.0f;
     float len = 0.0f, dist = 0.0f;
     if (inb) {
         const float s = ang[2 * v], c = ang[2 * v + 1];
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
     const int64_t pix = (int64_t)row * d.R + col;
     float r = 0.0f;
     if (!(len < d.vz * 1e-6f)) {
-        const float ax = (float)(sum * (double)dist);
+        const float ax = sum * dist;
         if constexpr (CMD == 0) r = ax;
         else r = ((float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix) - ax) / len;
     }
@@ -783,17 +785,43 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         }
     }
 
-    // this ray: [alpha_min, alpha_max], length and z slab as the per-ray walk
+    // this ray: its length as the per-ray walk forms it (0 for the rays the
+    // reference leaves empty), and its height in voxel units along the
+    // column's alpha, zeta(alpha) = (z1 + alpha (z2 - z1) - bz)/vz, as one FMA
+    // zeta is kept relative to the slab kb the ray enters in, so that it
+    // stays small and its rounding does not move the crossings of nearly
+    // level rays; kb4 = 4 kb re-bases the window offsets.
     FastWalk<ES, true> fw;
-    float lo = 0.0f, hi = 0.0f, az = __builtin_inff(), nfz = 0.0f, kz = 0.0f, bz0 = __builtin_inff(), sgzf = 0.0f;
     float len = 0.0f, dist = 0.0f;
-    int zi4 = 0, dzi4 = 0;   // z slab and its step, in bytes of the staged window
-    if (!cmiss && row < d.ZP && fw.init(d, s, c, col, row, src)) {
-        lo = fw.ac; hi = fw.amax; len = fw.len; dist = fw.dist;
-        az = fw.az; nfz = fw.nfz; kz = fw.kz; bz0 = fw.bz0; sgzf = fw.sgzf;
-        zi4 = ((int)nfz - (sgzf > 0.0f ? 1 : 0)) * 4;
-        dzi4 = (int)sgzf * 4;
+    float cz = 0.0f, z0c = 0.0f, icz = 0.0f;
+    int kb4 = 0;
+    {
+        const float z1 = d.zmin + (float)row * d.dv;
+        float sx, sy, z2;
+        ray_source(d, s, c, u, z1, src, sx, sy, z2);
+        cz = (z2 - z1) * d.ivz;
+        const float kb = floorf(fmaf(lxy, cz, (z1 - d.bz) * d.ivz));
+        kb4 = 4 * (int)kb;
+        z0c = fmaf(z1 - d.bz, d.ivz, -kb);
+        // alpha per unit of zeta (0 for a level ray, which never changes slab)
+        icz = fabsf(cz) > 1e-30f ? 1.0f / fabsf(cz) : 0.0f;
     }
+    // CMD 0 (ray sums of the autograd pair): the per-ray walk's own segments,
+    // so that the matched adjoint stays its exact transpose.  The lane clips
+    // the list to its [alpha_min, alpha_max] and tracks its next z-plane
+    // crossing fma(nfz, kz, bz0); zr4 is its slab relative to kb, times 4.
+    float lo = 0.0f, hi = 0.0f, az = __builtin_inff(), nfz = 0.0f, kz = 0.0f, bz0 = __builtin_inff(), sgzf = 0.0f;
+    int zr4 = 0, dzi4 = 0;
+    if (!cmiss && row < d.ZP && fw.init(d, s, c, col, row, src)) {
+        len = fw.len; dist = fw.dist;
+        if constexpr (CMD == 0) {
+            lo = fw.ac; hi = fw.amax;
+            az = fw.az; nfz = fw.nfz; kz = fw.kz; bz0 = fw.bz0; sgzf = fw.sgzf;
+            zr4 = ((int)nfz - (sgzf > 0.0f ? 1 : 0)) * 4 - kb4;
+            dzi4 = (int)sgzf * 4;
+        }
+    }
+    const bool up = cz >= 0.0f;
     const unsigned nbytes = MN * (unsigned)ZQ4 * 4u * (unsigned)ES;
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(fq, nbytes);
     // this thread's staged quads (the same in every batch): segment k of the
@@ -830,9 +858,68 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     __syncthreads();   // the list is complete
     const int nb = KP / COL_B;
     if (nb > 0) { issue(0); commit(std::integral_constant<int, 0>()); }
+    // the lower element of a segment's slab pair lies at most 4 NQ - 2 into
+    // its window (one voxel of margin each way by construction; clamped)
+    const unsigned wmax2 = (unsigned)(4 * NQ - 2) * 4u;
+    // Kahan-compensated FP32 total of the batches' FP32 sums
+    float sum = 0.0f, comp = 0.0f;
+    float as = lxy, zs = fmaf(lxy, cz, z0c);
+    // CMD 1 (SART corrections): a segment [as, ae] of the column's list lies
+    // in one x/y voxel; this ray
+    // enters it in slab k = floor(zeta(as)) and, as |dzeta| < 1 per segment
+    // (col_walk_nq), crosses at most one z plane in it.  The crossing part,
+    // in zeta units clamp(zeta(ae) - (k + 1), 0, 1) for a rising ray or
+    // clamp(k - zeta(ae), 0, 1) for a falling one, times alpha per unit of
+    // zeta, belongs to the next slab and the rest of the segment to slab k;
+    // both slabs are one LDS pair read.  Parts below or above the volume read
+    // the window's zeros, so no per-ray clipping is needed: the sum is the
+    // per-ray walk's to rounding.
+    auto segs = [&](auto upc, const char *sb, const float *ea, const int *ew) {
+        constexpr bool UP = decltype(upc)::value;
+        float ps = 0.0f;
+#pragma unroll
+        for (int k = 0; k < COL_B; ++k) {
+            const float ae = ea[k];
+            const float ze = fmaf(ae, cz, z0c);
+            const float kf = floorf(zs);
+            const float wx = (UP ? __builtin_amdgcn_fmed3f(ze - kf - 1.0f, 0.0f, 1.0f)
+                                 : __builtin_amdgcn_fmed3f(kf - ze, 0.0f, 1.0f)) * icz;
+            const float wy = (ae - as) - wx;
+            const int klo = (int)kf - (UP ? 0 : 1);
+            const unsigned a = min((unsigned)(klo * 4 + ew[k]), wmax2);
+            const float *pr = reinterpret_cast<const float *>(sb + k * COL_W * 4 + a);
+            const float f0 = pr[0], f1 = pr[1];
+            ps = fmaf(UP ? wy : wx, f0, ps);
+            ps = fmaf(UP ? wx : wy, f1, ps);
+            as = ae;
+            zs = ze;
+        }
+        return ps;
+    };
     const unsigned wmax = (unsigned)(4 * NQ - 1) * 4u;
-    double sum = 0.0;
-    float as = lo;
+    float asx = lo;
+    auto segs_exact = [&](const char *sb, const float *ea, const int *ew) {
+        float ps = 0.0f;
+#pragma unroll
+        for (int k = 0; k < COL_B; ++k) {
+            const float ae = __builtin_amdgcn_fmed3f(ea[k], lo, hi);
+            const float m = fminf(az, ae);
+            const float d0 = m - asx, d1 = ae - m;
+            const bool cr = az < ae;
+            // the slab before and after the crossing, as byte offsets in the
+            // window (an idle lane's slab may lie outside it: clamped, and its
+            // segment lengths are 0)
+            const unsigned r0 = min((unsigned)(zr4 + ew[k]), wmax);
+            zr4 += cr ? dzi4 : 0;
+            nfz += cr ? sgzf : 0.0f;
+            az = fmaf(nfz, kz, bz0);
+            const unsigned r1 = min((unsigned)(zr4 + ew[k]), wmax);
+            ps = fmaf(d0, *reinterpret_cast<const float *>(sb + k * COL_W * 4 + r0), ps);
+            ps = fmaf(d1, *reinterpret_cast<const float *>(sb + k * COL_W * 4 + r1), ps);
+            asx = ae;
+        }
+        return ps;
+    };
     // one batch from staging buffer BUF; the next batch's quads are loaded
     // meanwhile and go to the other buffer
     auto batch = [&](auto bufc, int b) {
@@ -840,39 +927,24 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         __syncthreads();   // batch b staged; batch b - 1's readers are done with the other buffer
         if (b + 1 < nb) issue(b + 1);
         const char *sb = reinterpret_cast<const char *>(stg + BUF * COL_B * COL_W);
-        float d0[COL_B], d1[COL_B], f0[COL_B], f1[COL_B], ea[COL_B];
+        float ea[COL_B];
         int ew[COL_B];
 #pragma unroll
         for (int k = 0; k < COL_B; k += 4) {
             const float4 a4 = *reinterpret_cast<const float4 *>(la + b * COL_B + k);
             const int4 w4 = *reinterpret_cast<const int4 *>(lw + b * COL_B + k);
             ea[k] = a4.x; ea[k + 1] = a4.y; ea[k + 2] = a4.z; ea[k + 3] = a4.w;
-            ew[k] = w4.x; ew[k + 1] = w4.y; ew[k + 2] = w4.z; ew[k + 3] = w4.w;
+            // this ray's slab kb at the window's byte 0
+            ew[k] = kb4 - w4.x; ew[k + 1] = kb4 - w4.y; ew[k + 2] = kb4 - w4.z; ew[k + 3] = kb4 - w4.w;
         }
-#pragma unroll
-        for (int k = 0; k < COL_B; ++k) {
-            const float ae = __builtin_amdgcn_fmed3f(ea[k], lo, hi);
-            const float m = fminf(az, ae);
-            d0[k] = m - as;
-            d1[k] = ae - m;
-            const bool cr = az < ae;
-            // the slab before and after the crossing, as byte offsets in the
-            // window (an idle lane's slab may lie outside it: clamped, and its
-            // segment lengths are 0)
-            const unsigned r0 = min((unsigned)(zi4 - ew[k]), wmax);
-            zi4 += cr ? dzi4 : 0;
-            nfz += cr ? sgzf : 0.0f;
-            az = fmaf(nfz, kz, bz0);
-            const unsigned r1 = min((unsigned)(zi4 - ew[k]), wmax);
-            f0[k] = *reinterpret_cast<const float *>(sb + k * COL_W * 4 + r0);
-            f1[k] = *reinterpret_cast<const float *>(sb + k * COL_W * 4 + r1);
-            as = ae;
-        }
-        float ps = d0[0] * f0[0];
-        ps = fmaf(d1[0], f1[0], ps);
-#pragma unroll
-        for (int k = 1; k < COL_B; ++k) { ps = fmaf(d0[k], f0[k], ps); ps = fmaf(d1[k], f1[k], ps); }
-        sum += (double)ps;
+        // rising and falling rays (all but one wave per column are uniform)
+        float ps;
+        if constexpr (CMD == 0) ps = segs_exact(sb, ea, ew);
+        else ps = up ? segs(std::true_type(), sb, ea, ew) : segs(std::false_type(), sb, ea, ew);
+        const float yk = ps - comp;
+        const float tk = sum + yk;
+        comp = (tk - sum) - yk;
+        sum = tk;
         if (b + 1 < nb) commit(std::integral_constant<int, 1 - BUF>());
     };
     for (int b = 0; b < nb; b += 2) {
@@ -884,7 +956,7 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     const int64_t pix = (int64_t)row * d.R + col;
     float r = 0.0f;
     if (!(len < d.vz * 1e-6f)) {
-        const float ax = (float)(sum * (double)dist);
+        const float ax = (sum - comp) * dist;
         if constexpr (CMD == 0) r = ax;
         else r = ((float)ldf(meas, ((int64_t)(v - mv0) * d.NS + src) * P + pix) - ax) / len;
     }
@@ -1183,9 +1255,10 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const flo
     const float TX0 = d.bx + (float)tx0 * d.vx, TX1 = d.bx + (float)min(tx0 + 16, d.M) * d.vx;
     const float TY0 = d.by + (float)ty0 * d.vy, TY1 = d.by + (float)min(ty0 + 16, d.N) * d.vy;
     const float TZ0 = d.bz + (float)iz0 * d.vz, TZ1 = d.bz + (float)(iz0 + nz) * d.vz;
-    double W[AZR];
+    // per-voxel sums over the images, Kahan-compensated FP32
+    float W[AZR], Wc[AZR];
 #pragma unroll
-    for (int k = 0; k < AZR; ++k) W[k] = 0.0;
+    for (int k = 0; k < AZR; ++k) { W[k] = 0.0f; Wc[k] = 0.0f; }
     const int64_t P = (int64_t)d.R * d.ZP;
     for (int img = 0; img < nimg; ++img) {
         const int q = img / d.NS, src = img - q * d.NS;
@@ -1293,14 +1366,16 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const flo
             else if (NC == 6) a = adj_rows<6>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
             else if (NC == 4) a = adj_rows<4>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
             else a = adj_rows<3>(sw, sra, srb, z0, z1, cbase, iz, xin, xout);
-            W[k] += (double)a;
+            const float y = a - Wc[k], t = W[k] + y;
+            Wc[k] = (t - W[k]) - y;
+            W[k] = t;
         }
     }
     if (!active) return;
 #pragma unroll
     for (int k = 0; k < AZR; ++k) {
         if (k >= nz) break;
-        St<SO>::st(vol, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, (float)W[k]);
+        St<SO>::st(vol, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, W[k] - Wc[k]);
     }
 }
 

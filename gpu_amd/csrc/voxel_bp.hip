@@ -429,45 +429,86 @@ __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const flo
 
 // Stochastic rounding of an updated voxel to 16-bit storage.  A SART step
 // adds ~1e-4 of a voxel's value, a fraction of an FP16 ulp; rounding to
-// nearest loses it with a bias that accumulates over iterations.  The dropped
-// significand bits (13 for FP16, 16 for BF16) get a dither, then the value is
-// truncated: it rounds away from zero with probability equal to the dropped
-// fraction, so the stored value is the exact one in expectation.  The dither
-// is the low significand bits of the voxel's own sums W and L (sums of ~10^2
-// products that change with every view), which is one instruction per
-// voxel, and keeps runs reproducible.  In the FP16 subnormal range
-// (|v| < 2^-14) the conversion drops more than 13 bits, so the dither there is
-// added in float as u * 2^-24 (one subnormal ulp) with u in [0, 1) from the
-// same bits; NaN and infinities are stored as they are.
-template <typename SV> __device__ __forceinline__ void st_sr(SV *p, int64_t i, float v, uint32_t r) {
+// nearest loses it with a bias that accumulates over iterations.  Stochastic
+// rounding keeps it in expectation: the value rounds up with probability
+// equal to its fraction of an ulp.  The random bits come from the voxel's own
+// sums W and L (sums of ~10^2 products that change with every view) through
+// sr_bits, which keeps runs reproducible.
+//  * FP16: |v| + u ulp16(v), u in [0, 1) from the low 16 bits, truncated
+//    towards zero by the packed round-to-zero conversion: the magnitude
+//    rounds up exactly when u >= 1 - fraction, probability `fraction` to
+//    within 2^-16, so the stored value is the exact one in expectation
+//    (also at powers of two, where truncation lands on the grid of v's own
+//    binade or the next).  ulp16(v) = 2^(e - 10) from v's exponent e, 2^-24
+//    in the subnormal range (|v| < 2^-14); the exponent is capped at 2^15,
+//    so infinities stay infinite and a NaN stays NaN; finite values beyond
+//    the FP16 range store 65504.  A value already on the FP16 grid never
+//    moves.
+//  * BF16: the 16 dropped bits get the dither, then the value is truncated
+//    (exact in expectation); a NaN keeps a quiet-NaN pattern instead of
+//    carrying into the sign bit.
+template <typename SV, typename I> __device__ __forceinline__ void st_sr(SV *p, I i, float v, uint32_t r) {
     St<SV>::st(p, i, v);
 }
-template <> __device__ __forceinline__ void st_sr<__half>(__half *p, int64_t i, float v, uint32_t r) {
-    const float a = fabsf(v);
-    const float wn = __uint_as_float(__float_as_uint(v) + (r & 0x1FFFu));
-    const float ws = v + __builtin_copysignf((float)(r & 0xFFFFu) * 0x1p-40f, v);
-    p[i] = __float2half_rz(a < 0x1p-14f ? ws : (a <= 65504.0f ? wn : v));
+template <typename I> __device__ __forceinline__ void st_sr(__half *p, I i, float v, uint32_t r) {
+    const uint32_t b = __float_as_uint(v);
+    // sign(v) 2^e, e clamped to [-14, 15]
+    const float e2 = __uint_as_float(min(max(b & 0x7F800000u, 0x38800000u), 0x47000000u) | (b & 0x80000000u));
+    const auto h2 = __builtin_amdgcn_cvt_pkrtz(fmaf((float)(r & 0xFFFFu) * 0x1p-26f, e2, v), 0.0f);
+    p[i] = __builtin_bit_cast(__half, h2[0]);
 }
-template <> __device__ __forceinline__ void st_sr<__hip_bfloat16>(__hip_bfloat16 *p, int64_t i, float v, uint32_t r) {
+template <typename I> __device__ __forceinline__ void st_sr(__hip_bfloat16 *p, I i, float v, uint32_t r) {
     const uint32_t u = __float_as_uint(v) + (r & 0xFFFFu);
-    // a NaN keeps a quiet-NaN pattern instead of carrying into the sign bit
     const unsigned short h = v == v ? (unsigned short)(u >> 16) : (unsigned short)0x7FC0u;
     p[i] = __builtin_bit_cast(__hip_bfloat16, h);
 }
+// The dither bits: the low significand bits of (W c) XOR (L c) for
+// c = 1/sqrt(2) rounded to float.  The packed-half engine's W and L are short
+// sums of exact FP16 products whose low bits are mostly zero, and under a
+// uniform correction W is nearly a fixed multiple of L, so their own low bits
+// are neither uniform nor independent; each product with c carries ~48
+// significant bits into its rounding, which makes the low bits of both vary
+// at every ulp.  Two multiplies and an XOR.  Over 20 SART-FISTA-TV
+// iterations at 512^3 / 1024 views this leaves FP16 storage 4.7x closer to
+// the FP32 path than a hash of the voxel index and view does, with no larger
+// mean drift; (W c + L) alone correlates with the rounded fraction and drifts
+// 10x more.
+__device__ __forceinline__ uint32_t sr_bits(float W, float L) {
+    return __float_as_uint(W * 0x1.6a09e6p-1f) ^ __float_as_uint(L * 0x1.6a09e6p-1f);
+}
+
+// The update thresholds on L as one float comparison: !(L < vx 1e-6) for
+// cone beam (V63/kernel_IterativeRecon_universal.cu:799) and L > vx 1e-3 in
+// double for multi-source (TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772),
+// which for a float L is L > (vx 1e-3 rounded down to float).  `scale`
+// carries L's units (the packed-half engine's kl).
+struct BpThresh {
+    float lth;
+    bool tb;
+    __device__ __forceinline__ BpThresh(const DGeo<float> &d, float scale) : tb(d.tb) {
+        if (!d.tb) lth = d.vx * 1e-6f * scale;
+        else {
+            const double t = (double)d.vx * 1e-3;
+            float f = (float)t;
+            if ((double)f > t) f = __uint_as_float(__float_as_uint(f) - 1u);   // next float towards 0 (f > 0)
+            lth = f * scale;
+        }
+    }
+    __device__ __forceinline__ bool pass(float L) const { return tb ? (L > lth) : !(L < lth); }
+};
 
 // SART update of one voxel from its accumulated W = sum l*c and L = sum l
 // (V63/kernel_IterativeRecon_universal.cu:799-806; TB threshold
-// TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772).  16-bit volumes store the
-// update with stochastic rounding.
-template <typename SV>
-__device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__ f, int64_t idx, float W, float L,
-                                          float beta, int mode) {
-    bool upd;
-    if (!d.tb) upd = !(L < d.vx * 1e-6f);
-    else upd = (double)L > (double)d.vx * 1e-3;
-    if (!upd) return;
-    if (mode == 0) St<SV>::st(f, idx, W);
-    else st_sr<SV>(f, idx, ldf(f, idx) + beta * W / L, __float_as_uint(W) ^ __float_as_uint(L));
+// TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772) in fast mode: W / L by
+// the hardware reciprocal (1 ulp), 16-bit volumes stored with stochastic
+// rounding.  W and L may carry a common scale (their ratio is unchanged);
+// wscale takes W back to length units for mode 0 (x = W).
+template <typename SV, typename I>
+__device__ __forceinline__ void bp_update(SV *__restrict__ f, I idx, float W, float L, float beta, int mode,
+                                          const BpThresh &th, float wscale) {
+    if (!th.pass(L)) return;
+    if (mode == 0) St<SV>::st(f, idx, W * wscale);
+    else st_sr(f, idx, fmaf(beta * W, frcp(L), ldf(f, idx)), sr_bits(W, L));
 }
 
 // ONE: the launch backprojects a single image (one view, one source), so each
@@ -478,11 +519,14 @@ __device__ __forceinline__ void bp_update(const DGeo<float> &d, SV *__restrict__
 // TR: the transposed z-quad layout [z/4][x][y][z%4] of the sweep's
 // y-directed views; the lanes then run along y (threadIdx.x -> y) so that a
 // wave's accesses stay contiguous.
+// (element index; the LDS-staged kernel runs on volumes of under 2^31
+// elements, bp_lds_fits)
 template <bool ZQ, bool TR = false>
-__device__ __forceinline__ int64_t vox_index(const DGeo<float> &d, int ix, int iy, int iz) {
-    if constexpr (ZQ && TR) return (((int64_t)(iz >> 2) * d.M + ix) * d.N + iy) * 4 + (iz & 3);
-    else if constexpr (ZQ) return (((int64_t)(iz >> 2) * d.N + iy) * d.M + ix) * 4 + (iz & 3);
-    else return ((int64_t)iz * d.N + iy) * d.M + ix;
+__device__ __forceinline__ uint32_t vox_index(const DGeo<float> &d, int ix, int iy, int iz) {
+    const uint32_t M = d.M, N = d.N;
+    if constexpr (ZQ && TR) return ((((uint32_t)iz >> 2) * M + (uint32_t)ix) * N + (uint32_t)iy) * 4u + ((uint32_t)iz & 3u);
+    else if constexpr (ZQ) return ((((uint32_t)iz >> 2) * N + (uint32_t)iy) * M + (uint32_t)ix) * 4u + ((uint32_t)iz & 3u);
+    else return ((uint32_t)iz * N + (uint32_t)iy) * M + (uint32_t)ix;
 }
 
 template <typename SV, bool ONE, bool H, bool ZQ, bool TR = false>
@@ -527,7 +571,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
     const float TZ0 = d.bz + (float)iz0 * d.vz, TZ1 = d.bz + (float)(iz0 + nz) * d.vz;
     // packed-half engine: alpha scale kl and its inverse
     const float vdiag = sqrtf(d.vx * d.vx + d.vy * d.vy + d.vz * d.vz);
-    const float kl = (d.dso - d.dod) / vdiag, ikl = vdiag / (d.dso - d.dod);
+    const float ivd = frcp(vdiag);
+    const float kl = (d.dso - d.dod) * ivd, ikl = vdiag * frcp(d.dso - d.dod);
+    // the packed-half engine's sums come out times kl: W/L is unchanged, the
+    // threshold on L and mode 0's W carry the factor
+    const BpThresh th(d, H ? kl : 1.0f);
+    const float wsc = H ? ikl : 1.0f;
     float W[ONE ? 1 : ZR], L[ONE ? 1 : ZR];
 #pragma unroll
     for (int k = 0; k < (ONE ? 1 : ZR); ++k) { W[k] = 0.0f; L[k] = 0.0f; }
@@ -638,7 +687,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                 hi[cc] = in ? hmin(hmax(ax0, ax1), hmax(ay0, ay1)) : -__builtin_inff();
             }
             // H: the column intervals in voxel-local alpha units, paired
-            const float nA = (d.dod - (cx * c + cy * s)) / vdiag;
+            const float nA = (d.dod - (cx * c + cy * s)) * ivd;
             unsigned lo2[NCMAX / 2], hi2[NCMAX / 2];
             if constexpr (H) {
 #pragma unroll
@@ -648,23 +697,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     hi2[p] = pk_rtz(fmaf(hi[2 * p], kl, nA), fmaf(hi[2 * p + 1], kl, nA));
                 }
             }
+            // the voxel's z faces from a float plane counter (one FMA each)
+            float zk = (float)iz0;
+            const float zrow = -d.zmin * d.idv;
 #pragma unroll 1
-            for (int k = 0; k < nz; ++k) {
-                const float cz = !d.tb ? d.bz + d.vz * 0.5f + (float)(iz0 + k) * d.vz
-                                       : d.bz + d.vz * (0.5f + (float)(iz0 + k));
+            for (int k = 0; k < nz; ++k, zk += 1.0f) {
+                const float zl = fmaf(zk, d.vz, d.bz), zh = zl + d.vz;
                 if (d.tb) {
+                    const float cz = d.bz + d.vz * (0.5f + (float)(iz0 + k));
                     const float upper = (cz + d.vz - d.zmin) * rk + d.zmin;
                     const float lower = (cz - d.vz - d.zmax) * rk + d.zmax;
                     if (szs < lower || szs > upper) continue;
                 }
-                const float zl = cz - hvz, zh = cz + hvz;
                 // w = z + r*(z - z_source) is monotone in r: the extremes of the
                 // eight corners come from the extreme ratios
                 const float el = zl - szs, eh = zh - szs;
                 const float wl = fmaf(el >= 0.0f ? rmin : rmax, el, zl);
                 const float wh = fmaf(eh >= 0.0f ? rmax : rmin, eh, zh);
-                int z0 = shadow_lo((wl - d.zmin) * d.idv);
-                int z1 = shadow_hi((wh - d.zmin) * d.idv);
+                int z0 = shadow_lo(fmaf(wl, d.idv, zrow));
+                int z1 = shadow_hi(fmaf(wh, d.idv, zrow));
                 if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
                 z0 = max(z0, R0) - R0;
                 z1 = min(z1, R1) - R0;
@@ -680,9 +731,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                     else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                 }
-                if constexpr (ONE)
-                    bp_update(d, f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), H ? wk * ikl : wk, H ? lk * ikl : lk, beta,
-                              mode);
+                if constexpr (ONE) bp_update(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), wk, lk, beta, mode, th, wsc);
                 else { W[k] = wk; L[k] = lk; }
             }
         }
@@ -692,8 +741,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
 #pragma unroll
         for (int k = 0; k < ZR; ++k) {
             if (k >= nz) break;
-            bp_update(d, f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), H ? W[k] * ikl : W[k], H ? L[k] * ikl : L[k],
-                      beta, mode);
+            bp_update(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), W[k], L[k], beta, mode, th, wsc);
         }
     }
 }
@@ -758,7 +806,7 @@ __global__ __launch_bounds__(256) void k_voxel_bp(DGeo<T> d, const T *__restrict
         const T x = (T)ldf(f, idx);
         // fast mode stores 16-bit updates with stochastic rounding, as the
         // LDS-staged and planar kernels do (exact mode rounds to nearest)
-        if constexpr (FAST) st_sr<SV>(f, idx, x + beta * W / L, __float_as_uint((float)W) ^ __float_as_uint((float)L));
+        if constexpr (FAST) st_sr(f, idx, x + beta * W / L, sr_bits((float)W, (float)L));
         else St<SV>::st(f, idx, x + beta * W / L);
     }
 }
@@ -782,6 +830,7 @@ __global__ __launch_bounds__(256) void k_voxel_bp_planar(DGeo<float> d, const fl
     const int ix = blockIdx.x * 64 + threadIdx.x;
     const int iy = blockIdx.y * 4 + threadIdx.y;
     const int iz0 = blockIdx.z * PZR;
+    const BpThresh th(d, 1.0f);
     if (ix >= d.M || iy >= d.N) return;
     const int nz = min(PZR, d.Z - iz0);
     // cylinder mask (:626-628)
@@ -837,7 +886,7 @@ __global__ __launch_bounds__(256) void k_voxel_bp_planar(DGeo<float> d, const fl
     }
 #pragma unroll
     for (int k = 0; k < PZR; ++k)
-        if (k < nz) bp_update(d, f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, W[k], L, beta, mode);
+        if (k < nz) bp_update(f, ((int64_t)(iz0 + k) * d.N + iy) * d.M + ix, W[k], L, beta, mode, th, 1.0f);
 }
 
 // Fan or parallel beam with the detector rows at the slice centres.

@@ -221,15 +221,21 @@ ct_status ct_adjoint_test(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
 ct_status ct_power_iteration(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
                              int32_t niter, uint64_t seed, double *lipschitz);
 
-/* Lipschitz estimate of the SART step: 2 x the largest eigenvalue of the
- * operator one SART update applies over all views, T = D^-1 B^T M^-1 A (A the
- * forward projection, M the rays' lengths, B^T the voxel-driven
- * backprojection, D its per-voxel chord sums), by niter power iterations with
- * FP64 norms; dtype is the arithmetic of the projections (F32 or F64).  The
- * SART-FISTA step is beta = 2 / lipschitz and the FGP argument
- * 2 lambda_TV / lipschitz, replacing the constant Lip_con of
- * V63/main_recon.cu:58,161-162. */
-ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dtype,
+/* Lipschitz estimate of the SART-FISTA step: 2 x the largest eigenvalue of
+ * the operator that one SART sweep with subsets of `subset` consecutive
+ * views applies to first order in its step,
+ *     K = sum over subsets S of D_S^-1 B_S^T M_S^-1 A_S
+ * (A_S the subset's forward projection, M_S its rays' lengths, B_S^T the
+ * voxel-driven backprojection, D_S its per-voxel chord sums), by niter power
+ * iterations with FP64 norms; dtype is the arithmetic of the projections
+ * (F32 or F64).  K has non-negative entries, so its largest eigenvalue is its
+ * spectral radius, with a non-negative eigenvector.  The step
+ * beta = 2 / lipschitz = 1 / rho(K) keeps a sweep inside FISTA's 1/L bound on
+ * the gradient step, and the FGP argument is 2 lambda_TV / lipschitz; they
+ * replace the constant Lip_con of V63/main_recon.cu:58,161-162.  subset =
+ * nviews gives the simultaneous operator (radius at most 1); view by view
+ * (subset 1) it grows with the number of views. */
+ct_status ct_sart_lipschitz(ct_context *ctx, const ct_geometry *g, ct_dtype dtype, int32_t subset,
                             int32_t niter, uint64_t seed, double *lipschitz);
 
 /* ---- FISTA / TV ------------------------------------------------------- */
