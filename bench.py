@@ -432,7 +432,7 @@ def main() -> int:
         vol = torch.zeros(g.vol_shape, device="cuda", dtype=sdt)
         ctx.sart_sweep(gs, vol, meas, beta=beta, subset=1)      # the sweep's kernels, as timed
         # the same number of views from 90 degrees on: rays along y, which the
-        # correction walks ray by ray (the views above take the column walk)
+        # sweep walks by column on its transposed z-quad copy
         from dataclasses import replace
         g90 = replace(gs, angle0_deg=90.0)
         ctx.sart_sweep(g90, vol, ctx.forward(g90, ball), beta=beta, subset=1)

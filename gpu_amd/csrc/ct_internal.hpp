@@ -5,6 +5,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 #include <cstdint>
+#include <type_traits>
 #include <cstddef>
 #include <string>
 #include <vector>
@@ -38,6 +39,21 @@ template <typename S> struct LdT { using type = float; };
 template <> struct LdT<double> { using type = double; };
 template <typename S> __device__ __forceinline__ typename LdT<S>::type ldf(const S *p, int64_t i) {
     return St<S>::ld(p, i);
+}
+
+// Raw register form of a loaded value: 16-bit types stay as their bits until
+// use, so a load issued ahead of its use is not waited for by an early
+// conversion (which would serialise its latency).
+template <typename D> struct RawOf { using type = D; };
+template <> struct RawOf<__half> { using type = unsigned short; };
+template <> struct RawOf<__hip_bfloat16> { using type = unsigned short; };
+template <typename D> __device__ __forceinline__ typename RawOf<D>::type ld_raw(const D *p, int64_t i) {
+    return reinterpret_cast<const typename RawOf<D>::type *>(p)[i];
+}
+template <typename T, typename D> __device__ __forceinline__ T from_raw(typename RawOf<D>::type r) {
+    if constexpr (std::is_same<D, __half>::value) return (T)(float)__builtin_bit_cast(_Float16, r);
+    else if constexpr (std::is_same<D, __hip_bfloat16>::value) return (T)__uint_as_float((unsigned)r << 16);
+    else return (T)r;
 }
 
 // hardware reciprocal and square root of the fast kernels

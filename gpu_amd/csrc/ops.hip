@@ -501,15 +501,18 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
     }
     // every load of a slice is issued one step before its use: the staged
     // duals (extended tile, zero outside the volume), b on the L extent and
-    // the owner's o, p, q
-    T gr[2], gs[2], gt[2], gb[2];
+    // the owner's o, p, q; held in their raw storage form until used
+    using RD = typename RawOf<D>::type;
+    using RS = typename RawOf<S>::type;
+    RD gr[2], gs[2], gt[2];
+    RS gb[2];
     auto fetch_stage = [&](int k) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             gr[u] = 0; gs[u] = 0; gt[u] = 0;
             if (soff[u] >= 0) {
                 const int64_t v = (int64_t)k * MN + soff[u];
-                gr[u] = ldd<T>(rA, v); gs[u] = ldd<T>(sA, v); gt[u] = ldd<T>(tA, v);
+                gr[u] = ld_raw(rA, v); gs[u] = ld_raw(sA, v); gt[u] = ld_raw(tA, v);
             }
         }
     };
@@ -518,21 +521,21 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int e = tid + u * NT;
-            if (e < FEX * FEY) { sr[bf][e] = gr[u]; ss[bf][e] = gs[u]; stt[bf][e] = gt[u]; }
+            if (e < FEX * FEY) {
+                sr[bf][e] = from_raw<T, D>(gr[u]); ss[bf][e] = from_raw<T, D>(gs[u]);
+                stt[bf][e] = from_raw<T, D>(gt[u]);
+            }
         }
     };
     auto fetch_b = [&](int k) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             gb[u] = 0;
-            if (loff[u] >= 0) {
-                const int64_t v = (int64_t)k * MN + loff[u];
-                if constexpr (std::is_same<S, double>::value) gb[u] = b[v]; else gb[u] = ldf(b, v);
-            }
+            if (loff[u] >= 0) gb[u] = ld_raw(b, (int64_t)k * MN + loff[u]);
         }
     };
     // L of slice k on the tile + right column + bottom row (k_fgp_L)
-    auto formL = [&](int k, const T (&bv)[2]) {
+    auto formL = [&](int k, const RS (&bv)[2]) {
         const int bf = k % 3, pf = (k + 2) % 3;   // slices k and k - 1
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
@@ -547,18 +550,18 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
                 const T sm = j > 0 ? ss[bf][c - FEX] : (T)0;
                 const T tm = k > 0 ? stt[pf][c] : (T)0;
                 const T tl = (sr[bf][c] - rm) + (ss[bf][c] - sm) + (stt[bf][c] - tm);
-                L = bv[u] - lambda * tl;
+                L = from_raw<T, S>(bv[u]) - lambda * tl;
             }
             sL[k & 1][e] = L;
         }
     };
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i < M && j < N;
-    T po = 0, pp = 0, pq = 0;
+    RD po = 0, pp = 0, pq = 0;
     auto fetch_own = [&](int k) {
         if (own) {
             const int64_t v = (int64_t)k * MN + (int64_t)j * M + i;
-            po = ldd<T>(o, v); pp = ldd<T>(p, v); pq = ldd<T>(q, v);
+            po = ld_raw(o, v); pp = ld_raw(p, v); pq = ld_raw(q, v);
         }
     };
     // prologue: slices k0 - 1 and k0 staged, L(k0) formed, slice k0 + 1's
@@ -573,8 +576,8 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
     for (int k = k0; k < k1; ++k) {
         const bool nx = k + 1 < Z;
         if (nx) put_stage(k + 1);
-        const T co = po, cp = pp, cq = pq;
-        T bn[2] = {gb[0], gb[1]};
+        const RD co = po, cp = pp, cq = pq;
+        const RS bn[2] = {gb[0], gb[1]};
         if (k + 2 < Z && k + 1 < k1) { fetch_stage(k + 2); fetch_b(k + 2); }
         if (k + 1 < k1) fetch_own(k + 1);
         __syncthreads();   // slice k + 1 staged; L(k) complete
@@ -593,11 +596,11 @@ __global__ __launch_bounds__(FGX * FGY) void k_fgp_step(const S *__restrict__ b,
         if (k < Z - 1) tt = tt + a12 * (Lv - sL[(k + 1) & 1][l]);
         const T nrm = sqrt(tr * tr + ts * ts + tt * tt);
         const T den = nrm > (T)1 ? nrm : (T)1;
-        if (i < M - 1) { const T on = tr / den; std_(rB, v, on + g * (on - co)); std_(o, v, on); }
+        if (i < M - 1) { const T on = tr / den; std_(rB, v, on + g * (on - from_raw<T, D>(co))); std_(o, v, on); }
         else std_(rB, v, r0);
-        if (j < N - 1) { const T pn = ts / den; std_(sB, v, pn + g * (pn - cp)); std_(p, v, pn); }
+        if (j < N - 1) { const T pn = ts / den; std_(sB, v, pn + g * (pn - from_raw<T, D>(cp))); std_(p, v, pn); }
         else std_(sB, v, s0);
-        if (k < Z - 1) { const T qn = tt / den; std_(tB, v, qn + g * (qn - cq)); std_(q, v, qn); }
+        if (k < Z - 1) { const T qn = tt / den; std_(tB, v, qn + g * (qn - from_raw<T, D>(cq))); std_(q, v, qn); }
         else std_(tB, v, t0);
     }
 }

@@ -274,6 +274,11 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
 // over the subset's correction images: per pixel (c*|S-P|, |S-P|), per column
 // (qx, qy, 1/ex, 1/ey), per row and source (qz, 1/ez).  The pixel, slab-plane
 // and source coordinates use the reference's arithmetic (see voxel_wl_fast).
+// PK (the packed-half engine of reduced-precision volumes): the pixel entry
+// holds the column pair (i, i + 1) as half pairs, (c*|S-P|, c*|S-P|) and
+// (|S-P|, |S-P|) rounded to nearest (zero past the last column), so that the
+// engine stages a footprint row by plain 8-byte copies.
+template <bool PK>
 __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__restrict__ ang,
                                                  const float *__restrict__ corr, int first, int stride,
                                                  float2 *__restrict__ pix, float4 *__restrict__ colT,
@@ -287,16 +292,34 @@ __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__r
     const float szs = d.tb ? (float)(d.sz0 + (float)src * d.sdz) : d.sz0;
     const float sxw = d.sx * c - d.sy * s;
     const float syw = d.sx * s + d.sy * c;
-    const float u = d.ymin + (float)i * d.du;
-    const float qx = d.dod * c - u * s;
-    const float qy = d.dod * s + u * c;
     const float qz = d.zmin + (float)j * d.dv;
-    const float ex = sxw - qx, ey = syw - qy, ez = szs - qz;
-    const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+    const float ez = szs - qz;
     const int64_t P = (int64_t)d.R * d.ZP;
     const int64_t p = (int64_t)img * P + (int64_t)j * d.R + i;
-    pix[p] = make_float2(corr[p] * dist, dist);
-    if (j == 0 && src == 0) colT[(int64_t)q * d.R + i] = make_float4(qx, qy, 1.0f / ex, 1.0f / ey);
+    // (c*|S-P|, |S-P|) of column i2 of this row; its detector point (qx, qy)
+    // and 1/ex, 1/ey for the column table
+    auto pixel = [&](int i2, float4 &ct) {
+        const float u = d.ymin + (float)i2 * d.du;
+        const float qx = d.dod * c - u * s;
+        const float qy = d.dod * s + u * c;
+        const float ex = sxw - qx, ey = syw - qy;
+        const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+        ct = make_float4(qx, qy, 1.0f / ex, 1.0f / ey);
+        return make_float2(corr[p + (i2 - i)] * dist, dist);
+    };
+    float4 ct, cn;
+    const float2 a = pixel(i, ct);
+    if constexpr (PK) {
+        const float2 b = i + 1 < d.R ? pixel(i + 1, cn) : make_float2(0.0f, 0.0f);
+        const unsigned wx = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)a.x) |
+                            ((unsigned)__builtin_bit_cast(unsigned short, (_Float16)b.x) << 16);
+        const unsigned wy = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)a.y) |
+                            ((unsigned)__builtin_bit_cast(unsigned short, (_Float16)b.y) << 16);
+        pix[p] = make_float2(__uint_as_float(wx), __uint_as_float(wy));
+    } else {
+        pix[p] = a;
+    }
+    if (j == 0 && src == 0) colT[(int64_t)q * d.R + i] = ct;
     if (i == 0) rowT[(int64_t)img * d.ZP + j] = make_float2(qz, 1.0f / ez);
 }
 
@@ -322,11 +345,6 @@ constexpr int NCMAX = 8;
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned pk_rtz(float a, float b) {
     return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
-}
-__device__ __forceinline__ unsigned pk_rne(float a, float b) {
-    const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)a);
-    const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)b);
-    return lo | (hi << 16);
 }
 __device__ __forceinline__ unsigned pk_min(unsigned a, unsigned b) {
     unsigned r;
@@ -481,20 +499,21 @@ __device__ __forceinline__ uint32_t sr_bits(float W, float L) {
 // cone beam (V63/kernel_IterativeRecon_universal.cu:799) and L > vx 1e-3 in
 // double for multi-source (TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:772),
 // which for a float L is L > (vx 1e-3 rounded down to float).  `scale`
-// carries L's units (the packed-half engine's kl).
+// carries L's units (the packed-half engine's kl).  Both are one comparison
+// !(L < lth): for multi-source, L > t is L >= t' with t' the next float
+// above t (L is a sum of clamped spans times finite lengths, never NaN).
 struct BpThresh {
     float lth;
-    bool tb;
-    __device__ __forceinline__ BpThresh(const DGeo<float> &d, float scale) : tb(d.tb) {
+    __device__ __forceinline__ BpThresh(const DGeo<float> &d, float scale) {
         if (!d.tb) lth = d.vx * 1e-6f * scale;
         else {
             const double t = (double)d.vx * 1e-3;
             float f = (float)t;
             if ((double)f > t) f = __uint_as_float(__float_as_uint(f) - 1u);   // next float towards 0 (f > 0)
-            lth = f * scale;
+            lth = __uint_as_float(__float_as_uint(f * scale) + 1u);            // next float above (f * scale > 0)
         }
     }
-    __device__ __forceinline__ bool pass(float L) const { return tb ? (L > lth) : !(L < lth); }
+    __device__ __forceinline__ bool pass(float L) const { return !(L < lth); }
 };
 
 // SART update of one voxel from its accumulated W = sum l*c and L = sum l
@@ -509,6 +528,14 @@ __device__ __forceinline__ void bp_update(SV *__restrict__ f, I idx, float W, fl
     if (!th.pass(L)) return;
     if (mode == 0) St<SV>::st(f, idx, W * wscale);
     else st_sr(f, idx, fmaf(beta * W, frcp(L), ldf(f, idx)), sr_bits(W, L));
+}
+// the same with the voxel's old value loaded ahead (raw storage form; mode 1)
+template <typename SV, typename I>
+__device__ __forceinline__ void bp_update_x(SV *__restrict__ f, I idx, typename RawOf<SV>::type xr, float W, float L,
+                                            float beta, int mode, const BpThresh &th, float wscale) {
+    if (!th.pass(L)) return;
+    if (mode == 0) St<SV>::st(f, idx, W * wscale);
+    else st_sr(f, idx, fmaf(beta * W, frcp(L), from_raw<float, SV>(xr)), sr_bits(W, L));
 }
 
 // ONE: the launch backprojects a single image (one view, one source), so each
@@ -643,18 +670,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                 // (up to the row pitch) are zero
                 const int cl = tid & 63;
                 for (int r = tid >> 6; r < nrow; r += 4) {
+                    // (H: the entries are already column pairs of halves)
                     const float2 *g = pimg + (int64_t)(R0 + r) * d.R + C0;
-                    const float2 a = cl < ncw ? g[cl] : make_float2(0.0f, 0.0f);
-                    if constexpr (H) {
-                        const float2 b = cl + 1 < ncw ? g[cl + 1] : make_float2(0.0f, 0.0f);
-                        sph[r * LP + cl] = make_uint2(pk_rne(a.x, b.x), pk_rne(a.y, b.y));
-                    } else {
-                        sp[r * LP + cl] = a;
-                    }
-                    if (cl < LP - LW) {
-                        if constexpr (H) sph[r * LP + LW + cl] = make_uint2(0u, 0u);
-                        else sp[r * LP + LW + cl] = make_float2(0.0f, 0.0f);
-                    }
+                    sp[r * LP + cl] = cl < ncw ? g[cl] : make_float2(0.0f, 0.0f);
+                    if (cl < LP - LW) sp[r * LP + LW + cl] = make_float2(0.0f, 0.0f);
                 }
             }
             if (tid < LP) sc[tid] = tid < ncw ? cimg[C0 + tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -697,25 +716,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     hi2[p] = pk_rtz(fmaf(hi[2 * p], kl, nA), fmaf(hi[2 * p + 1], kl, nA));
                 }
             }
-            // the voxel's z faces from a float plane counter (one FMA each)
-            float zk = (float)iz0;
+            // The voxels' z faces: the planes from a float plane counter (one
+            // FMA each), each plane evaluated once for the two voxels it
+            // bounds.  Its shadow height w = z + r (z - z_source) is monotone
+            // in the corner ratio r, so the extremes over the eight corners
+            // come from the extreme ratios: min(w(rmin), w(rmax)) is the lowest
+            // shadow of a lower face, the max the highest of an upper face.
             const float zrow = -d.zmin * d.idv;
+            float zk = (float)iz0;
+            float zh = fmaf(zk, d.vz, d.bz);
+            int z0n;   // shadow of the next voxel's lower face
+            {
+                const float e = zh - szs;
+                z0n = shadow_lo(fmaf(hmin(fmaf(rmin, e, zh), fmaf(rmax, e, zh)), d.idv, zrow));
+            }
 #pragma unroll 1
-            for (int k = 0; k < nz; ++k, zk += 1.0f) {
-                const float zl = fmaf(zk, d.vz, d.bz), zh = zl + d.vz;
+            for (int k = 0; k < nz; ++k) {
+                const float zl = zh;
+                zk += 1.0f;
+                zh = fmaf(zk, d.vz, d.bz);
+                const float e = zh - szs;
+                const float wa = fmaf(rmin, e, zh), wb = fmaf(rmax, e, zh);
+                int z0 = z0n;
+                int z1 = shadow_hi(fmaf(hmax(wa, wb), d.idv, zrow));
+                z0n = shadow_lo(fmaf(hmin(wa, wb), d.idv, zrow));
+                // single image: the voxel's old value is loaded here, its
+                // latency hidden behind the chord sums (mode 0 leaves it
+                // unset and unused: an initialising move would wait for the
+                // previous voxel's load)
+                typename RawOf<SV>::type xr;
+                if constexpr (ONE) {
+                    if (mode != 0) xr = ld_raw(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k));
+                }
                 if (d.tb) {
                     const float cz = d.bz + d.vz * (0.5f + (float)(iz0 + k));
                     const float upper = (cz + d.vz - d.zmin) * rk + d.zmin;
                     const float lower = (cz - d.vz - d.zmax) * rk + d.zmax;
                     if (szs < lower || szs > upper) continue;
                 }
-                // w = z + r*(z - z_source) is monotone in r: the extremes of the
-                // eight corners come from the extreme ratios
-                const float el = zl - szs, eh = zh - szs;
-                const float wl = fmaf(el >= 0.0f ? rmin : rmax, el, zl);
-                const float wh = fmaf(eh >= 0.0f ? rmax : rmin, eh, zh);
-                int z0 = shadow_lo(fmaf(wl, d.idv, zrow));
-                int z1 = shadow_hi(fmaf(wh, d.idv, zrow));
                 if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
                 z0 = max(z0, R0) - R0;
                 z1 = min(z1, R1) - R0;
@@ -731,7 +769,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                     else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                 }
-                if constexpr (ONE) bp_update(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), wk, lk, beta, mode, th, wsc);
+                if constexpr (ONE) bp_update_x(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), xr, wk, lk, beta, mode, th, wsc);
                 else { W[k] = wk; L[k] = lk; }
             }
         }
@@ -946,12 +984,12 @@ static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *
     float4 *colT = (float4 *)(tab + pix_b);
     float2 *rowT = (float2 *)(tab + pix_b + col_b);
     ProfScope ps(ctx, CT_K_VOXEL_BP);
-    hipLaunchKernelGGL(k_bp_prep, dim3((d.R + 255) / 256, d.ZP, nimg), dim3(256), 0, ctx->stream, d, ang,
+    // reduced-precision storage takes the packed-half chord engine
+    constexpr bool H = !std::is_same<SV, float>::value;
+    hipLaunchKernelGGL(k_bp_prep<H>, dim3((d.R + 255) / 256, d.ZP, nimg), dim3(256), 0, ctx->stream, d, ang,
                        (const float *)corr, first, stride, pix, colT, rowT);
     const int zr = nimg == 1 ? zrun<true>() : zrun<false>();
     dim3 lgrid((d.M + BPX - 1) / BPX, (d.N + BPY - 1) / BPY, (d.Z + zr - 1) / zr);
-    // reduced-precision storage takes the packed-half chord engine
-    constexpr bool H = !std::is_same<SV, float>::value;
     if (nimg == 1)
         hipLaunchKernelGGL((k_voxel_bp_lds<SV, true, H, ZQ, TR>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
                            (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
