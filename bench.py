@@ -473,24 +473,28 @@ def main() -> int:
         f"voxel bp {bp_ms / bp_n:.3f} ms x {bp_n}, GRays/s {grays:.3f}")
 
     V, P = g.nvox, g.nrows * g.ncols
-    kernels = {
-        "forward": (fwd_ms / fwd_n, 4 * V + 4 * P + 4 * P),     # read f, read b, write c
-        "voxel_bp": (bp_ms / bp_n, 4 * P + 2 * 4 * V),           # read c, read+write f
-    }
-    dom = max(kernels, key=lambda k: prof32[k][0])
-    avg_ms, abytes = kernels[dom]
-    achieved = abytes / (avg_ms * 1e-3) / 1e9
-    # HBM bytes and VALU instructions per launch of the same kernels, from the
-    # committed rocprofv3 counter passes (they cannot be read from inside this
-    # process); the live time turns the VALU count into an issue-rate fraction
-    traffic, valu_frac = None, None
+    # HBM bytes and VALU instructions per launch of the sweep kernels, from
+    # the committed rocprofv3 counter passes (they cannot be read from inside
+    # this process); the live time turns the VALU count into an issue-rate
+    # fraction
     tf = ROOT / "profiles" / "traffic.json"
-    if tf.exists():
-        tj = json.loads(tf.read_text())
-        traffic = tj.get("hbm_bytes_per_launch", {}).get(dom)
-        vi = tj.get("valu_insts_per_launch", {}).get(dom)
-        if vi:
-            valu_frac = round(vi / (VALU_ISSUE_PER_S * avg_ms * 1e-3), 4)
+    tj = json.loads(tf.read_text()) if tf.exists() else {}
+
+    def roofline(name, avg_ms, abytes, key):
+        """Roofline of one sweep kernel class (one launch = one view)."""
+        achieved = abytes / (avg_ms * 1e-3) / 1e9
+        vi = tj.get("valu_insts_per_launch", {}).get(key)
+        return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": tj.get("hbm_bytes_per_launch", {}).get(key),
+                "algorithmic_bytes_per_launch": abytes, "ms_per_launch": round(avg_ms, 4),
+                "valu_issue_frac": round(vi / (VALU_ISSUE_PER_S * avg_ms * 1e-3), 4) if vi else None}
+
+    # algorithmic bytes per view: the correction reads f and b and writes c;
+    # the update reads c and reads and writes f
+    rl32 = {"forward": roofline("forward", fwd_ms / fwd_n, 4 * V + 4 * P + 4 * P, "forward"),
+            "voxel_bp": roofline("voxel_bp", bp_ms / bp_n, 4 * P + 2 * 4 * V, "voxel_bp")}
+    dom = max(rl32, key=lambda k: rl32[k]["ms_per_launch"])
 
     result = {
         "metric": METRIC,
@@ -516,9 +520,8 @@ def main() -> int:
         },
         "s_per_iter": {"f32": round(ms32 / 1e3, 4)},
         "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof32.items() if v[1]},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": abytes, "valu_issue_frac": valu_frac},
+        "roofline": rl32[dom],
+        "roofline_sweep_kernels": {"f32": rl32},
     }
 
     # secondary figures on the same inputs: FDK (Hann) over all views, and the
@@ -580,6 +583,13 @@ def main() -> int:
                 "grays": round(grays_r, 4), "speedup_vs_f32": round(ms32 / ms_r, 3),
                 "rmse_vs_f32_per_mm": rmse,
                 "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof_r.items() if v[1]}}
+            if tag == "f16":
+                # 16-bit volume and measurements, FP32 corrections
+                result["roofline_sweep_kernels"]["f16"] = {
+                    "forward": roofline("forward", prof_r["forward"][0] / prof_r["forward"][1],
+                                        2 * V + 2 * P + 4 * P, "forward_f16"),
+                    "voxel_bp": roofline("voxel_bp", prof_r["voxel_bp"][0] / prof_r["voxel_bp"][1],
+                                         4 * P + 2 * 2 * V, "voxel_bp_f16")}
             log(f"{tag.upper()}: {ms_r / 1e3:.3f} s/iter, GRays/s {grays_r:.3f}, speedup {ms32 / ms_r:.3f}, "
                 f"rmse vs f32 {rmse:.3e}")
             del meas_r, x_r

@@ -644,12 +644,15 @@ constexpr int COL_NQ = 48;      // largest staged window, in z-quads
 constexpr int COL_W = 4 * COL_NQ;   // LDS floats per staged window
 
 // LDS: two staging buffers of COL_B windows in a static array (fixed address
-// and stride, so every staged read has a constant offset); the list in
-// dynamic LDS as three arrays of at most M + N - 1 entries padded to a
-// multiple of COL_B: alpha, the window's byte offset in a staged row, and
-// (x/y byte offset, first staged quad) for the staging loads
-static inline size_t col_list_entries(int M, int N) { return (size_t)(M + N + COL_B); }
-static inline size_t col_dyn_bytes(int M, int N) { return 16 * col_list_entries(M, N); }
+// and stride, so every staged read has a constant offset; FP32 elements, or
+// the volume's own halves for FP16); the list in dynamic LDS as three arrays
+// of at most M + N - 1 entries padded to a multiple of COL_B: alpha, the
+// window's first quad times the staged bytes per quad (its byte offset in a
+// staged row), and the x/y byte offset for the staging loads.
+// (a multiple of 4, so the alpha and window arrays stay 16-byte aligned for
+// their vector reads)
+static inline size_t col_list_entries(int M, int N) { return (size_t)((M + N + COL_B + 3) & ~3); }
+static inline size_t col_dyn_bytes(int M, int N) { return 12 * col_list_entries(M, N); }
 static inline size_t col_lds_bytes(int M, int N) {
     return sizeof(float) * 2 * COL_B * COL_W + col_dyn_bytes(M, N);
 }
@@ -676,6 +679,11 @@ template <> __device__ __forceinline__ float4 load_quad<__hip_bfloat16>(__amdgpu
                        __uint_as_float(y & 0xffff0000u));
 }
 
+// Element type of the staged windows: FP16 volumes stage their halves as
+// loaded (the walk's FMAs take FP16 operands directly), the others FP32.
+template <typename SV> struct StageOf { using type = float; };
+template <> struct StageOf<__half> { using type = _Float16; };
+
 // TR: the volume is in the transposed z-quad layout [z/4][x][y][z%4], for
 // views whose rays run along y (the staged quads of consecutive segments then
 // share cache lines, as they do along x in the plain z-quad layout).
@@ -684,12 +692,15 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
                                                           const SV *__restrict__ fq, SO *__restrict__ out,
                                                           const SM *__restrict__ meas, int first, int stride,
                                                           int mv0, int z0, int NQ) {
-    __shared__ __align__(16) float stg[2 * COL_B * COL_W];
+    using SE = typename StageOf<SV>::type;
+    constexpr int SB = (int)sizeof(SE);                // staged bytes per voxel
+    constexpr int QSH = SB == 4 ? 4 : 3;               // log2 of the staged bytes per quad
+    __shared__ __align__(16) SE stg[2 * COL_B * COL_W];
     extern __shared__ __align__(16) unsigned char col_smem[];
-    const int LE = d.M + d.N + COL_B;
+    const int LE = (d.M + d.N + COL_B + 3) & ~3;   // col_list_entries
     float *const la = reinterpret_cast<float *>(col_smem);             // alpha
-    int *const lw = reinterpret_cast<int *>(col_smem + 4 * LE);          // window offset (bytes)
-    uint2 *const ls = reinterpret_cast<uint2 *>(col_smem + 8 * LE);      // (x/y byte offset, first quad)
+    int *const lw = reinterpret_cast<int *>(col_smem + 4 * LE);          // first quad x 4 SB
+    unsigned *const ls = reinterpret_cast<unsigned *>(col_smem + 8 * LE); // x/y byte offset
     constexpr int ES = (int)sizeof(SV);
     const int tid = (int)threadIdx.x;
     const int3 t = xcd_tile();
@@ -778,8 +789,8 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
                 else { a = YA(j); ++j; }
                 const int ql = QLO(ap, a);
                 la[e] = a;
-                lw[e] = ql * 16;
-                ls[e] = make_uint2(off, (unsigned)ql);
+                lw[e] = ql * (1 << QSH);
+                ls[e] = off;
                 ap = a;
             }
         }
@@ -790,18 +801,18 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     // column's alpha, zeta(alpha) = (z1 + alpha (z2 - z1) - bz)/vz, as one FMA
     // zeta is kept relative to the slab kb the ray enters in, so that it
     // stays small and its rounding does not move the crossings of nearly
-    // level rays; kb4 = 4 kb re-bases the window offsets.
+    // level rays; kbb = SB kb (bytes) re-bases the window offsets.
     FastWalk<ES, true> fw;
     float len = 0.0f, dist = 0.0f;
     float cz = 0.0f, z0c = 0.0f, icz = 0.0f;
-    int kb4 = 0;
+    int kbb = 0;
     {
         const float z1 = d.zmin + (float)row * d.dv;
         float sx, sy, z2;
         ray_source(d, s, c, u, z1, src, sx, sy, z2);
         cz = (z2 - z1) * d.ivz;
         const float kb = floorf(fmaf(lxy, cz, (z1 - d.bz) * d.ivz));
-        kb4 = 4 * (int)kb;
+        kbb = SB * (int)kb;
         z0c = fmaf(z1 - d.bz, d.ivz, -kb);
         // alpha per unit of zeta (0 for a level ray, which never changes slab)
         icz = fabsf(cz) > 1e-30f ? 1.0f / fabsf(cz) : 0.0f;
@@ -809,16 +820,16 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     // CMD 0 (ray sums of the autograd pair): the per-ray walk's own segments,
     // so that the matched adjoint stays its exact transpose.  The lane clips
     // the list to its [alpha_min, alpha_max] and tracks its next z-plane
-    // crossing fma(nfz, kz, bz0); zr4 is its slab relative to kb, times 4.
+    // crossing fma(nfz, kz, bz0); zrb is its slab relative to kb, times SB.
     float lo = 0.0f, hi = 0.0f, az = __builtin_inff(), nfz = 0.0f, kz = 0.0f, bz0 = __builtin_inff(), sgzf = 0.0f;
-    int zr4 = 0, dzi4 = 0;
+    int zrb = 0, dzb = 0;
     if (!cmiss && row < d.ZP && fw.init(d, s, c, col, row, src)) {
         len = fw.len; dist = fw.dist;
         if constexpr (CMD == 0) {
             lo = fw.ac; hi = fw.amax;
             az = fw.az; nfz = fw.nfz; kz = fw.kz; bz0 = fw.bz0; sgzf = fw.sgzf;
-            zr4 = ((int)nfz - (sgzf > 0.0f ? 1 : 0)) * 4 - kb4;
-            dzi4 = (int)sgzf * 4;
+            zrb = ((int)nfz - (sgzf > 0.0f ? 1 : 0)) * SB - kbb;
+            dzb = (int)sgzf * SB;
         }
     }
     const bool up = cz >= 0.0f;
@@ -838,14 +849,21 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     }
     // a quad outside the volume reads as zeros: its index is clamped to -1 or
     // ZQ4, whose offsets lie past the buffer's end (the volume is under 2 GiB)
-    float4 sreg[2];
+    using SR = typename std::conditional<SB == 2, uint2, float4>::type;
+    SR sreg[2];
     auto issue = [&](int b) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             if (sk[p] >= 0) {
-                const uint2 en = ls[b * COL_B + sk[p]];
-                const int qz = min(max((int)en.y + sq[p], -1), ZQ4);
-                sreg[p] = load_quad<SV>(rs, en.x + (unsigned)qz * QS);
+                const int e = b * COL_B + sk[p];
+                const int qz = min(max((lw[e] >> QSH) + sq[p], -1), ZQ4);
+                const uint32_t off = ls[e] + (unsigned)qz * QS;
+                if constexpr (SB == 2) {
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+                    sreg[p] = make_uint2(v[0], v[1]);
+                } else {
+                    sreg[p] = load_quad<SV>(rs, off);
+                }
             }
         }
     };
@@ -853,14 +871,14 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         constexpr int buf = decltype(bufc)::value;
 #pragma unroll
         for (int p = 0; p < 2; ++p)
-            if (sk[p] >= 0) *reinterpret_cast<float4 *>(stg + (buf * COL_B * COL_W + so[p])) = sreg[p];
+            if (sk[p] >= 0) *reinterpret_cast<SR *>(stg + (buf * COL_B * COL_W + so[p])) = sreg[p];
     };
     __syncthreads();   // the list is complete
     const int nb = KP / COL_B;
     if (nb > 0) { issue(0); commit(std::integral_constant<int, 0>()); }
     // the lower element of a segment's slab pair lies at most 4 NQ - 2 into
     // its window (one voxel of margin each way by construction; clamped)
-    const unsigned wmax2 = (unsigned)(4 * NQ - 2) * 4u;
+    const unsigned wmax2 = (unsigned)((4 * NQ - 2) * SB);
     // Kahan-compensated FP32 total of the batches' FP32 sums
     float sum = 0.0f, comp = 0.0f;
     float as = lxy, zs = fmaf(lxy, cz, z0c);
@@ -886,9 +904,9 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
                                  : __builtin_amdgcn_fmed3f(kf - ze, 0.0f, 1.0f)) * icz;
             const float wy = (ae - as) - wx;
             const int klo = (int)kf - (UP ? 0 : 1);
-            const unsigned a = min((unsigned)(klo * 4 + ew[k]), wmax2);
-            const float *pr = reinterpret_cast<const float *>(sb + k * COL_W * 4 + a);
-            const float f0 = pr[0], f1 = pr[1];
+            const unsigned a = min((unsigned)(klo * SB + ew[k]), wmax2);
+            const SE *pr = reinterpret_cast<const SE *>(sb + k * COL_W * SB + a);
+            const float f0 = (float)pr[0], f1 = (float)pr[1];
             ps = fmaf(UP ? wy : wx, f0, ps);
             ps = fmaf(UP ? wx : wy, f1, ps);
             as = ae;
@@ -896,7 +914,7 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         }
         return ps;
     };
-    const unsigned wmax = (unsigned)(4 * NQ - 1) * 4u;
+    const unsigned wmax = (unsigned)((4 * NQ - 1) * SB);
     float asx = lo;
     auto segs_exact = [&](const char *sb, const float *ea, const int *ew) {
         float ps = 0.0f;
@@ -909,13 +927,13 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
             // the slab before and after the crossing, as byte offsets in the
             // window (an idle lane's slab may lie outside it: clamped, and its
             // segment lengths are 0)
-            const unsigned r0 = min((unsigned)(zr4 + ew[k]), wmax);
-            zr4 += cr ? dzi4 : 0;
+            const unsigned r0 = min((unsigned)(zrb + ew[k]), wmax);
+            zrb += cr ? dzb : 0;
             nfz += cr ? sgzf : 0.0f;
             az = fmaf(nfz, kz, bz0);
-            const unsigned r1 = min((unsigned)(zr4 + ew[k]), wmax);
-            ps = fmaf(d0, *reinterpret_cast<const float *>(sb + k * COL_W * 4 + r0), ps);
-            ps = fmaf(d1, *reinterpret_cast<const float *>(sb + k * COL_W * 4 + r1), ps);
+            const unsigned r1 = min((unsigned)(zrb + ew[k]), wmax);
+            ps = fmaf(d0, (float)*reinterpret_cast<const SE *>(sb + k * COL_W * SB + r0), ps);
+            ps = fmaf(d1, (float)*reinterpret_cast<const SE *>(sb + k * COL_W * SB + r1), ps);
             asx = ae;
         }
         return ps;
@@ -935,7 +953,7 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
             const int4 w4 = *reinterpret_cast<const int4 *>(lw + b * COL_B + k);
             ea[k] = a4.x; ea[k + 1] = a4.y; ea[k + 2] = a4.z; ea[k + 3] = a4.w;
             // this ray's slab kb at the window's byte 0
-            ew[k] = kb4 - w4.x; ew[k + 1] = kb4 - w4.y; ew[k + 2] = kb4 - w4.z; ew[k + 3] = kb4 - w4.w;
+            ew[k] = kbb - w4.x; ew[k + 1] = kbb - w4.y; ew[k + 2] = kbb - w4.z; ew[k + 3] = kbb - w4.w;
         }
         // rising and falling rays (all but one wave per column are uniform)
         float ps;
@@ -1441,17 +1459,25 @@ static int col_walk_nq(const DGeo<float> &d) {
     if (!(dzmax / (double)d.vz < 0.9 * inplane / (std::sqrt(2.0) * vxy))) return 0;
     const double nq = std::ceil((span + 8.0) / 4.0);
     if (!(nq >= 1.0) || nq > COL_NQ || nq * COL_B > 2.0 * COL_ROWS) return 0;
-    if (col_lds_bytes(d.M, d.N) > 65536) return 0;
+    if (col_lds_bytes(d.M, d.N) + 4 * col_list_entries(d.M, d.N) > 65536) return 0;
     return (int)nq;
 }
 
+// Dynamic LDS of a launch.  FP32 volumes allocate 4 more bytes per list
+// entry than the list uses, which holds them at five workgroups per CU at
+// 512^2 (0.464 ms per view at 512^3 / 1024^2, against 0.485 ms with six: the
+// staging loads of six contend); 16-bit volumes stage half the bytes and run
+// six per CU (0.379 against 0.407 ms).
+template <typename SV> static size_t col_dyn_lds(int M, int N) {
+    return col_dyn_bytes(M, N) + (sizeof(SV) == 4 ? 4 * col_list_entries(M, N) : 0);
+}
 template <typename SV, typename SO, typename SM, int CMD, bool TR = false>
 static void launch_col(ct_context *ctx, const DGeo<float> &d, int nq, const float *ang, const void *volq, void *out,
                        const void *meas, int first, int stride, int mv0, int nimg) {
     for (int z0 = 0; z0 < nimg; z0 += 65535) {
         const int nz = nimg - z0 < 65535 ? nimg - z0 : 65535;
         dim3 grid(d.R, (d.ZP + COL_ROWS - 1) / COL_ROWS, nz);
-        hipLaunchKernelGGL((k_project_col<SV, SO, SM, CMD, TR>), grid, dim3(COL_ROWS), col_dyn_bytes(d.M, d.N),
+        hipLaunchKernelGGL((k_project_col<SV, SO, SM, CMD, TR>), grid, dim3(COL_ROWS), col_dyn_lds<SV>(d.M, d.N),
                            ctx->stream, d, ang, (const SV *)volq, (SO *)out, (const SM *)meas, first, stride, mv0,
                            z0, nq);
     }
