@@ -450,12 +450,14 @@ template <int ES, bool ZQ = false> struct FastWalk {
 constexpr int FAST_B = 16;  // steps per batch: their gathers are in flight together
 
 // Fast-mode ray sum: batches of FAST_B steps whose loads are issued together;
-// each batch is summed in FP32 and the batch sums accumulate in FP64.
+// each batch is summed in FP32 and the batch sums accumulate in FP32 with
+// Kahan compensation.
 // Volumes under 4 GiB are read with range-checked buffer loads on 32-bit
 // byte offsets; larger ones use 64-bit addresses with an explicit range test.
 // The batch loop of the fast ray sum over an initialised walker: batches of
 // FAST_B steps whose loads are issued together, each batch summed in FP32 and
-// the batch sums accumulated in FP64.  Returns the sum of alpha-length x value
+// the batch sums accumulated Kahan-compensated in FP32.  Returns the sum of
+// alpha-length x value
 // (times dist = the ray sum).  Volumes under 4 GiB are read with
 // range-checked buffer loads on 32-bit byte offsets; larger ones use 64-bit
 // addresses with an explicit range test.
@@ -633,11 +635,12 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
 // |dz|/vz < max(|dx|/vx, |dy|/vy) along every ray (the host checks a bound);
 // the part after the crossing reads the next z voxel.  The segments are those
 // of the per-ray walk (same alphas, same voxels; a tie there is one step, here
-// a zero-length segment), and the sums keep its FP32-batch / FP64-total
-// accumulation.  Per segment ~18 VALU (staging included), two LDS reads and
-// no per-lane gather, against ~25 VALU and one scattered gather per step of
-// the per-ray walk.  The launchers use it for views whose rays run along x
-// (see col_walk_view).
+// a zero-length segment), and the sums keep its FP32 batches with a
+// Kahan-compensated FP32 total.  Per segment ~14 VALU (staging included), one
+// LDS pair read and no per-lane gather, against ~25 VALU and one scattered
+// gather per step of the per-ray walk.  Ray-sum batches use it for views
+// whose rays run along x (see col_walk_view); the SART sweep for every view,
+// on a transposed copy of its volume for views whose rays run along y.
 constexpr int COL_ROWS = 256;   // rays per workgroup: rows of one column
 constexpr int COL_B = 8;        // list segments per staged batch
 constexpr int COL_NQ = 48;      // largest staged window, in z-quads
@@ -1242,8 +1245,8 @@ __device__ __forceinline__ float adj_rows(const float *__restrict__ sw, const fl
 
 // one lane per (x, y) and its AZR z-voxels; a 16x16 (x, y) tile per
 // workgroup; per image the tile's detector footprint of weights is staged in
-// LDS.  Each view's per-voxel sum (FP32, ~10 terms) is added into an FP64
-// total; the volume is written once.
+// LDS.  Each view's per-voxel sum (FP32, ~10 terms) is added into an FP32
+// total with Kahan compensation; the volume is written once.
 template <typename SO>
 __global__ __launch_bounds__(256) void k_adjoint_gather(DGeo<float> d, const float *__restrict__ ang, int view0,
                                                         int nimg, const float *__restrict__ pixw,
