@@ -108,7 +108,7 @@ def accuracy(ctx, g, x_true, meas32, iters, beta, lam, niter):
     return out
 
 
-def adam_multisource(ctx, iters: int, storage: str) -> dict:
+def adam_multisource(ctx, iters: int, storage: str, ref_iters: int = 0) -> dict:
     """The multi-source gradient workload at its own size: 256^3 at 0.75 mm,
     48 point sources 3 mm apart on the axis, 720 views of 1024 x 128 at
     0.5 mm (34560 images, 4.5e9 rays); Adam (lr 1e-3) on
@@ -117,7 +117,12 @@ def adam_multisource(ctx, iters: int, storage: str) -> dict:
     re-walked instead of storing per-ray intermediates), volume stored in
     `storage` with FP32 master weights, from zero.  b = A x_true of the seeded
     phantom (FP32).  Reports seconds per iteration (HIP events), the loss per
-    iteration and the distance to the phantom."""
+    iteration and the distance to the phantom.  With `ref_iters`, the same
+    recurrence first runs that many iterations entirely in FP64 (the GPU's
+    FP64 instantiation of the projector pair, FP64 master weights and
+    measurements) and the run is compared with it after as many iterations:
+    an FP64 CPU reconstruction at this size would take days, and the 24^3
+    test compares with the FP64 CPU recurrence."""
     import torch
     from gpu_amd.autograd import project, tv_eps
     from gpu_amd.geometry import multisource_256
@@ -126,33 +131,58 @@ def adam_multisource(ctx, iters: int, storage: str) -> dict:
     sdt = {"bf16": torch.bfloat16, "f16": torch.float16, "f32": torch.float32}[storage]
     x_true = phantom(g, seed=0, scale=2.0, device="cuda")
     b = ctx.forward(g, x_true)
-    x = torch.zeros(g.vol_shape, device="cuda", requires_grad=True)
-    opt = torch.optim.Adam([x], lr=1e-3)
     lam, eps = 1e-3, 1e-8
-    losses, times = [], []
-    for k in range(iters):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        opt.zero_grad()
-        xs = x.to(sdt)
-        r = project(xs, g, ctx, out_dtype=torch.float32) - b
-        loss = 0.5 * torch.linalg.vector_norm(r, dtype=torch.float64) ** 2 + lam * tv_eps(xs, eps, ctx)
-        loss.backward()
-        opt.step()
-        e1.record()
-        torch.cuda.synchronize()
-        losses.append(float(loss))
-        times.append(e0.elapsed_time(e1) / 1e3)
-        log(f"adam {storage}: iteration {k + 1} loss {losses[-1]:.6e} {times[-1]:.2f} s")
-        del r, loss, xs
+
+    def run(n, store, master, meas, tag, snap_at=0):
+        x = torch.zeros(g.vol_shape, device="cuda", dtype=master, requires_grad=True)
+        opt = torch.optim.Adam([x], lr=1e-3)
+        losses, times, snap = [], [], None
+        for k in range(n):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            opt.zero_grad()
+            xs = x.to(store)
+            r = project(xs, g, ctx, out_dtype=meas.dtype) - meas
+            loss = 0.5 * torch.linalg.vector_norm(r, dtype=torch.float64) ** 2 + lam * tv_eps(xs, eps, ctx)
+            loss.backward()
+            opt.step()
+            e1.record()
+            torch.cuda.synchronize()
+            losses.append(float(loss))
+            times.append(e0.elapsed_time(e1) / 1e3)
+            log(f"adam {tag}: iteration {k + 1} loss {losses[-1]:.6e} {times[-1]:.2f} s")
+            del r, loss, xs
+            if k + 1 == snap_at:
+                snap = x.detach().double().clone()
+        return x.detach(), losses, times, snap
+
+    ref = None
+    if ref_iters > 0:
+        b64 = b.double()
+        x64, loss64, t64, _ = run(ref_iters, torch.float64, torch.float64, b64, "f64")
+        del b64
+        torch.cuda.empty_cache()
+        ref = (x64.double(), loss64, t64)
+    x, losses, times, snap = run(iters, sdt, torch.float32, b, storage, snap_at=ref_iters)
     xt = x_true.double()
-    d = x.detach().double() - xt
-    return {"workload": "multi-source Adam, 256^3 x 0.75 mm, 48 sources x 720 views of 1024 x 128, lr 1e-3, "
-                        "0.5||Ax-b||^2 + 1e-3 TV_eps",
-            "storage": storage, "iterations": iters,
-            "s_per_iteration": sum(times[1:]) / max(len(times) - 1, 1), "first_iteration_s": times[0],
-            "loss": losses, "rmse_vs_phantom": float(torch.sqrt((d * d).mean())),
-            "rel_l2_vs_phantom": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(xt))}
+    d = x.double() - xt
+    out = {"workload": "multi-source Adam, 256^3 x 0.75 mm, 48 sources x 720 views of 1024 x 128, lr 1e-3, "
+                       "0.5||Ax-b||^2 + 1e-3 TV_eps",
+           "storage": storage, "iterations": iters,
+           "s_per_iteration": sum(times[1:]) / max(len(times) - 1, 1), "first_iteration_s": times[0],
+           "loss": losses, "rmse_vs_phantom": float(torch.sqrt((d * d).mean())),
+           "rel_l2_vs_phantom": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(xt))}
+    if ref is not None and snap is not None:
+        x64, loss64, t64 = ref
+        dd = snap - x64
+        out["vs_f64"] = {
+            "iterations": ref_iters, "rmse": float(torch.sqrt((dd * dd).mean())),
+            "rel_l2": float(torch.linalg.vector_norm(dd) / torch.linalg.vector_norm(x64)),
+            "max_rel_loss_diff": max(abs(a - c) / abs(c) for a, c in zip(losses[:ref_iters], loss64)),
+            "f64_s_per_iteration": sum(t64) / len(t64),
+            "reference": "the same Adam recurrence in FP64 on the GPU (FP64 projector pair, master weights and "
+                         "measurements)"}
+    return out
 
 
 def cpu_model() -> str:
@@ -195,11 +225,20 @@ def cpu_baseline(g, meas_host, beta, lam, niter, sweep_views, full=False):
     out["sample"] = (f"first {sweep_views} views of the 512^3 / 1024-view SART sweep (correction + voxel update), "
                      f"FP32, {dt:.1f} s, OpenMP {threads} threads")
     sweep_s = dt / sweep_views * g.nviews
-    # TV-FGP and FISTA at 512^3
-    nfgp = niter if full else 2
-    t0 = time.perf_counter()
-    orc.tv_fgp(v, lam, nfgp, prec=32)
-    fgp_s = (time.perf_counter() - t0) / nfgp * niter
+    # TV-FGP and FISTA at 512^3 (sampled: 2 and 4 inner iterations, the
+    # difference giving the cost per inner iteration apart from the set-up)
+    if full:
+        t0 = time.perf_counter()
+        orc.tv_fgp(v, lam, niter, prec=32)
+        fgp_s = time.perf_counter() - t0
+    else:
+        t0 = time.perf_counter()
+        orc.tv_fgp(v, lam, 2, prec=32)
+        t2 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.tv_fgp(v, lam, 4, prec=32)
+        t4 = time.perf_counter() - t0
+        fgp_s = t2 + (t4 - t2) / 2 * (niter - 2)
     xp = np.zeros_like(v)
     t0 = time.perf_counter()
     orc.fista_momentum(v, xp, 0.5, prec=32)
@@ -208,7 +247,7 @@ def cpu_baseline(g, meas_host, beta, lam, niter, sweep_views, full=False):
         "sweep_s_extrapolated": round(sweep_s, 1), "tv_fgp_s": round(fgp_s, 2), "fista_s": round(fista_s, 3),
         "s_per_iter_extrapolated": round(sweep_s + fgp_s + fista_s, 1),
         "note": f"sweep extrapolated x{g.nviews / sweep_views:.0f} from {sweep_views} views"
-                + ("" if full else f"; TV-FGP({niter}) extrapolated from {nfgp} inner iterations")}
+                + ("" if full else f"; TV-FGP({niter}) extrapolated from runs of 2 and 4 inner iterations")}
     # FDK of 4 views
     g4 = g.with_views(4)
     t0 = time.perf_counter()
@@ -231,6 +270,32 @@ def cpu_baseline(g, meas_host, beta, lam, niter, sweep_views, full=False):
                              "s_per_iter": round(med / iters, 2),
                              "grays": round(g1.rays * iters / med / 1e9, 5)}
     return out
+
+
+def plain_sart_128_gpu(ctx) -> dict:
+    """The CPU-runnable workload on the GPU: 128^3 at 1 mm, 180 views of 256^2,
+    ten SART iterations (relaxation 1, interleaved subsets of 20 views) from
+    zero on the phantom's clean projections, FP32 (the CPU baseline runs the
+    same)."""
+    import torch
+    from gpu_amd.geometry import plain_sart_128
+    from gpu_amd.phantom import phantom
+    g1 = plain_sart_128()
+    x1 = phantom(g1, seed=0, device="cuda")
+    b1 = ctx.forward(g1, x1)
+    v = torch.zeros(g1.vol_shape, device="cuda")
+    ctx.sart_sweep(g1, v, b1, beta=1.0, subset=20, order="interleaved")   # warm-up
+    v.zero_()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        ctx.sart_sweep(g1, v, b1, beta=1.0, subset=20, order="interleaved")
+    e1.record()
+    torch.cuda.synchronize()
+    s = e0.elapsed_time(e1) / 1e3
+    return {"iterations": 10, "s": round(s, 4), "s_per_iter": round(s / 10, 5),
+            "grays": round(g1.rays * 10 / s / 1e9, 4)}
 
 
 def style_figures(ctx, g, x32) -> dict:
@@ -383,6 +448,9 @@ def main() -> int:
                     help="multi-source gradient mode: N Adam iterations of the 256^3 / 48-source / 720-view "
                          "workload (see --adam-storage); prints one JSON line, then exits")
     ap.add_argument("--adam-storage", choices=("bf16", "f16", "f32"), default="bf16")
+    ap.add_argument("--adam-ref", type=int, default=0,
+                    help="with --adam: first run this many iterations of the same recurrence in FP64 and report "
+                         "the run's distance to it after as many iterations")
     ap.add_argument("--beta", type=float, default=0.0,
                     help="SART relaxation; 0 (default) takes 2/Lip from 30 power iterations of the operator one "
                          "view-by-view sweep applies")
@@ -412,7 +480,7 @@ def main() -> int:
     beta = args.beta if args.beta > 0 else 0.0625
     ctx = Context(0)
     if args.adam:
-        print(json.dumps(adam_multisource(ctx, args.adam, args.adam_storage)), flush=True)
+        print(json.dumps(adam_multisource(ctx, args.adam, args.adam_storage, args.adam_ref)), flush=True)
         return 0
     if args.pmc_sample:
         # counter-collection mode: a cheap input (one ellipsoid-free ball) so the
@@ -559,6 +627,8 @@ def main() -> int:
                   "matched adjoint of the batch"}
     log(f"autograd pair: forward {pa['forward'][0] / nsamp:.3f} ms + adjoint {pa['adjoint'][0] / nsamp:.3f} ms per view")
     ctx.profile(False)
+    result["plain_sart_128"] = plain_sart_128_gpu(ctx)
+    log(f"128^3 SART: {result['plain_sart_128']['s_per_iter']} s per iteration")
     result["beam_styles"] = style_figures(ctx, g, x32)
     log("beam styles: " + ", ".join(f"{k} {v['grays']} GRays/s" for k, v in result["beam_styles"].items()
                                      if isinstance(v, dict)))

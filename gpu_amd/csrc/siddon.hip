@@ -644,7 +644,10 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
 constexpr int COL_ROWS = 256;   // rays per workgroup: rows of one column
 constexpr int COL_B = 8;        // list segments per staged batch
 constexpr int COL_NQ = 48;      // largest staged window, in z-quads
-constexpr int COL_W = 4 * COL_NQ;   // LDS floats per staged window
+// LDS elements per staged window: COL_NQ quads and one quad of padding, so
+// that the windows of a batch's segments start on different banks (the
+// staging writes of eight segments' same quad are then conflict-free)
+constexpr int COL_W = 4 * COL_NQ + 4;
 
 // LDS: two staging buffers of COL_B windows in a static array (fixed address
 // and stride, so every staged read has a constant offset; FP32 elements, or
@@ -841,14 +844,18 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     // this thread's staged quads (the same in every batch): segment k of the
     // batch and quad qi of its window (sk = -1: none), and where they go
     const int nst = COL_B * NQ;   // quads staged per batch (at most 2 * COL_ROWS)
+    // Segment-fastest: lanes 8q .. 8q + 7 load quad q of the batch's eight
+    // segments.  Consecutive segments are mostly neighbours along the
+    // layout's fast axis, whose quads share a 128-byte line, so a wave's load
+    // touches ~8-16 lines instead of one per lane (the quads of one segment
+    // lie a z-quad plane, M N quads, apart).
     int sk[2], sq[2], so[2];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         const int sidx = tid + p * COL_ROWS;
-        const int k = sidx / NQ;
-        sk[p] = sidx < nst ? k : -1;
-        sq[p] = sidx - k * NQ;
-        so[p] = k * COL_W + 4 * sq[p];
+        sk[p] = sidx < nst ? sidx % COL_B : -1;
+        sq[p] = sidx / COL_B;
+        so[p] = (sidx % COL_B) * COL_W + 4 * sq[p];
     }
     // a quad outside the volume reads as zeros: its index is clamped to -1 or
     // ZQ4, whose offsets lie past the buffer's end (the volume is under 2 GiB)
@@ -1462,17 +1469,25 @@ static int col_walk_nq(const DGeo<float> &d) {
     if (!(dzmax / (double)d.vz < 0.9 * inplane / (std::sqrt(2.0) * vxy))) return 0;
     const double nq = std::ceil((span + 8.0) / 4.0);
     if (!(nq >= 1.0) || nq > COL_NQ || nq * COL_B > 2.0 * COL_ROWS) return 0;
-    if (col_lds_bytes(d.M, d.N) + 4 * col_list_entries(d.M, d.N) > 65536) return 0;
+    // (the padding of col_dyn_lds keeps a workgroup under 33 KiB)
+    if (col_lds_bytes(d.M, d.N) > 65536) return 0;
     return (int)nq;
 }
 
-// Dynamic LDS of a launch.  FP32 volumes allocate 4 more bytes per list
-// entry than the list uses, which holds them at five workgroups per CU at
-// 512^2 (0.464 ms per view at 512^3 / 1024^2, against 0.485 ms with six: the
-// staging loads of six contend); 16-bit volumes stage half the bytes and run
-// six per CU (0.379 against 0.407 ms).
+// Dynamic LDS of a launch: the list, padded so that a workgroup holds at
+// least col_lds_floor bytes in all, which caps the workgroups per CU.
+// Measured per view at 512^3 / 1024^2 (a one-view launch is 4096
+// workgroups): FP32 volumes 0.457 ms at four per CU against 0.464 at five and
+// 0.485 at six; FP16 0.331 ms at four against 0.376 at seven; BF16 (windows
+// staged as FP32) 0.378 ms at six against 0.436 at four.
+template <typename SV> constexpr size_t col_lds_floor() {
+    return std::is_same<SV, __hip_bfloat16>::value ? 0 : 32 * 1024 + 512;   // > 160 KiB / 5
+}
 template <typename SV> static size_t col_dyn_lds(int M, int N) {
-    return col_dyn_bytes(M, N) + (sizeof(SV) == 4 ? 4 * col_list_entries(M, N) : 0);
+    const size_t stat = sizeof(typename StageOf<SV>::type) * 2 * COL_B * COL_W + 16;
+    const size_t want = col_lds_floor<SV>();
+    const size_t list = col_dyn_bytes(M, N);
+    return stat + list >= want ? list : want - stat;
 }
 template <typename SV, typename SO, typename SM, int CMD, bool TR = false>
 static void launch_col(ct_context *ctx, const DGeo<float> &d, int nq, const float *ang, const void *volq, void *out,
