@@ -642,21 +642,30 @@ __global__ __launch_bounds__(256) void k_project_zq(DGeo<float> d, const float *
 // whose rays run along x (see col_walk_view); the SART sweep for every view,
 // on a transposed copy of its volume for views whose rays run along y.
 constexpr int COL_ROWS = 256;   // rays per workgroup: rows of one column
-constexpr int COL_B = 8;        // list segments per staged batch
+constexpr int COL_B = 16;       // list padding: a multiple of every staged batch (col_batch)
 constexpr int COL_NQ = 48;      // largest staged window, in z-quads
 // LDS elements per staged window: COL_NQ quads and one quad of padding, so
 // that the windows of a batch's segments start on different banks (the
 // staging writes of eight segments' same quad are then conflict-free)
 constexpr int COL_W = 4 * COL_NQ + 4;
+// List segments per staged batch: 16 (one barrier per 16 segments, and a
+// staging load's eight-lane groups cover 16 segments), or 8 for BF16
+// volumes, whose FP32 windows at 16 would leave four workgroups per CU
+// instead of six (BF16 correction per view at 512^3 / 1024^2: 0.327 ms at 8,
+// 0.359 at 16; FP32 0.382 -> 0.342 ms and FP16 0.286 -> 0.261 ms from 8 to 16).
+template <typename SV> constexpr int col_batch() { return std::is_same<SV, __hip_bfloat16>::value ? 8 : 16; }
+// staged quads per thread and batch
+template <typename SV> constexpr int col_sp() { return (col_batch<SV>() * COL_NQ + COL_ROWS - 1) / COL_ROWS; }
 
-// LDS: two staging buffers of COL_B windows in a static array (fixed address
-// and stride, so every staged read has a constant offset; FP32 elements, or
-// the volume's own halves for FP16); the list in dynamic LDS as three arrays
-// of at most M + N - 1 entries padded to a multiple of COL_B: alpha, the
-// window's first quad times the staged bytes per quad (its byte offset in a
-// staged row), and the x/y byte offset for the staging loads.
-// (a multiple of 4, so the alpha and window arrays stay 16-byte aligned for
-// their vector reads)
+// LDS: two staging buffers of col_batch windows in a static array (fixed
+// address and stride, so every staged read has a constant offset; FP32
+// elements, or the volume's own halves for FP16); the list in dynamic LDS as
+// three arrays of at most M + N - 1 entries padded to a multiple of COL_B:
+// alpha, the window's first quad times the staged bytes per quad (its byte
+// offset in a staged row), and the x/y byte offset for the staging loads.
+// (The entry count is a multiple of 4, so the alpha and window arrays stay
+// 16-byte aligned for their vector reads; col_lds_bytes bounds the largest
+// workgroup.)
 static inline size_t col_list_entries(int M, int N) { return (size_t)((M + N + COL_B + 3) & ~3); }
 static inline size_t col_dyn_bytes(int M, int N) { return 12 * col_list_entries(M, N); }
 static inline size_t col_lds_bytes(int M, int N) {
@@ -701,7 +710,8 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     using SE = typename StageOf<SV>::type;
     constexpr int SB = (int)sizeof(SE);                // staged bytes per voxel
     constexpr int QSH = SB == 4 ? 4 : 3;               // log2 of the staged bytes per quad
-    __shared__ __align__(16) SE stg[2 * COL_B * COL_W];
+    constexpr int CB = col_batch<SV>(), SP = col_sp<SV>();
+    __shared__ __align__(16) SE stg[2 * CB * COL_W];
     extern __shared__ __align__(16) unsigned char col_smem[];
     const int LE = (d.M + d.N + COL_B + 3) & ~3;   // col_list_entries
     float *const la = reinterpret_cast<float *>(col_smem);             // alpha
@@ -843,29 +853,29 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(fq, nbytes);
     // this thread's staged quads (the same in every batch): segment k of the
     // batch and quad qi of its window (sk = -1: none), and where they go
-    const int nst = COL_B * NQ;   // quads staged per batch (at most 2 * COL_ROWS)
+    const int nst = CB * NQ;   // quads staged per batch (at most SP * COL_ROWS)
     // Segment-fastest: lanes 8q .. 8q + 7 load quad q of the batch's eight
     // segments.  Consecutive segments are mostly neighbours along the
     // layout's fast axis, whose quads share a 128-byte line, so a wave's load
     // touches ~8-16 lines instead of one per lane (the quads of one segment
     // lie a z-quad plane, M N quads, apart).
-    int sk[2], sq[2], so[2];
+    int sk[SP], sq[SP], so[SP];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < SP; ++p) {
         const int sidx = tid + p * COL_ROWS;
-        sk[p] = sidx < nst ? sidx % COL_B : -1;
-        sq[p] = sidx / COL_B;
-        so[p] = (sidx % COL_B) * COL_W + 4 * sq[p];
+        sk[p] = sidx < nst ? sidx % CB : -1;
+        sq[p] = sidx / CB;
+        so[p] = (sidx % CB) * COL_W + 4 * sq[p];
     }
     // a quad outside the volume reads as zeros: its index is clamped to -1 or
     // ZQ4, whose offsets lie past the buffer's end (the volume is under 2 GiB)
     using SR = typename std::conditional<SB == 2, uint2, float4>::type;
-    SR sreg[2];
+    SR sreg[SP];
     auto issue = [&](int b) {
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < SP; ++p) {
             if (sk[p] >= 0) {
-                const int e = b * COL_B + sk[p];
+                const int e = b * CB + sk[p];
                 const int qz = min(max((lw[e] >> QSH) + sq[p], -1), ZQ4);
                 const uint32_t off = ls[e] + (unsigned)qz * QS;
                 if constexpr (SB == 2) {
@@ -880,11 +890,11 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     auto commit = [&](auto bufc) {
         constexpr int buf = decltype(bufc)::value;
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-            if (sk[p] >= 0) *reinterpret_cast<SR *>(stg + (buf * COL_B * COL_W + so[p])) = sreg[p];
+        for (int p = 0; p < SP; ++p)
+            if (sk[p] >= 0) *reinterpret_cast<SR *>(stg + (buf * CB * COL_W + so[p])) = sreg[p];
     };
     __syncthreads();   // the list is complete
-    const int nb = KP / COL_B;
+    const int nb = KP / CB;
     if (nb > 0) { issue(0); commit(std::integral_constant<int, 0>()); }
     // the lower element of a segment's slab pair lies at most 4 NQ - 2 into
     // its window (one voxel of margin each way by construction; clamped)
@@ -906,7 +916,7 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         constexpr bool UP = decltype(upc)::value;
         float ps = 0.0f;
 #pragma unroll
-        for (int k = 0; k < COL_B; ++k) {
+        for (int k = 0; k < CB; ++k) {
             const float ae = ea[k];
             const float ze = fmaf(ae, cz, z0c);
             const float kf = floorf(zs);
@@ -929,7 +939,7 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     auto segs_exact = [&](const char *sb, const float *ea, const int *ew) {
         float ps = 0.0f;
 #pragma unroll
-        for (int k = 0; k < COL_B; ++k) {
+        for (int k = 0; k < CB; ++k) {
             const float ae = __builtin_amdgcn_fmed3f(ea[k], lo, hi);
             const float m = fminf(az, ae);
             const float d0 = m - asx, d1 = ae - m;
@@ -954,13 +964,13 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
         constexpr int BUF = decltype(bufc)::value;
         __syncthreads();   // batch b staged; batch b - 1's readers are done with the other buffer
         if (b + 1 < nb) issue(b + 1);
-        const char *sb = reinterpret_cast<const char *>(stg + BUF * COL_B * COL_W);
-        float ea[COL_B];
-        int ew[COL_B];
+        const char *sb = reinterpret_cast<const char *>(stg + BUF * CB * COL_W);
+        float ea[CB];
+        int ew[CB];
 #pragma unroll
-        for (int k = 0; k < COL_B; k += 4) {
-            const float4 a4 = *reinterpret_cast<const float4 *>(la + b * COL_B + k);
-            const int4 w4 = *reinterpret_cast<const int4 *>(lw + b * COL_B + k);
+        for (int k = 0; k < CB; k += 4) {
+            const float4 a4 = *reinterpret_cast<const float4 *>(la + b * CB + k);
+            const int4 w4 = *reinterpret_cast<const int4 *>(lw + b * CB + k);
             ea[k] = a4.x; ea[k + 1] = a4.y; ea[k + 2] = a4.z; ea[k + 3] = a4.w;
             // this ray's slab kb at the window's byte 0
             ew[k] = kbb - w4.x; ew[k + 1] = kbb - w4.y; ew[k + 2] = kbb - w4.z; ew[k + 3] = kbb - w4.w;
@@ -1468,7 +1478,7 @@ static int col_walk_nq(const DGeo<float> &d) {
     const double vxy = std::max((double)d.vx, (double)d.vy);
     if (!(dzmax / (double)d.vz < 0.9 * inplane / (std::sqrt(2.0) * vxy))) return 0;
     const double nq = std::ceil((span + 8.0) / 4.0);
-    if (!(nq >= 1.0) || nq > COL_NQ || nq * COL_B > 2.0 * COL_ROWS) return 0;
+    if (!(nq >= 1.0) || nq > COL_NQ) return 0;
     // (the padding of col_dyn_lds keeps a workgroup under 33 KiB)
     if (col_lds_bytes(d.M, d.N) > 65536) return 0;
     return (int)nq;
@@ -1484,7 +1494,7 @@ template <typename SV> constexpr size_t col_lds_floor() {
     return std::is_same<SV, __hip_bfloat16>::value ? 0 : 32 * 1024 + 512;   // > 160 KiB / 5
 }
 template <typename SV> static size_t col_dyn_lds(int M, int N) {
-    const size_t stat = sizeof(typename StageOf<SV>::type) * 2 * COL_B * COL_W + 16;
+    const size_t stat = sizeof(typename StageOf<SV>::type) * 2 * col_batch<SV>() * COL_W + 16;
     const size_t want = col_lds_floor<SV>();
     const size_t list = col_dyn_bytes(M, N);
     return stat + list >= want ? list : want - stat;
