@@ -277,7 +277,10 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
 // PK (the packed-half engine of reduced-precision volumes): the pixel entry
 // holds the column pair (i, i + 1) as half pairs, (c*|S-P|, c*|S-P|) and
 // (|S-P|, |S-P|) rounded to nearest (zero past the last column), so that the
-// engine stages a footprint row by plain 8-byte copies.
+// engine stages a footprint row by plain 8-byte copies.  Finite values are
+// clamped to the FP16 range first: a correction beyond it (a ray that grazes
+// the volume has a tiny length to divide by) would otherwise turn the voxel's
+// sums into infinities or NaN.
 template <bool PK>
 __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__restrict__ ang,
                                                  const float *__restrict__ corr, int first, int stride,
@@ -311,10 +314,12 @@ __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__r
     const float2 a = pixel(i, ct);
     if constexpr (PK) {
         const float2 b = i + 1 < d.R ? pixel(i + 1, cn) : make_float2(0.0f, 0.0f);
-        const unsigned wx = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)a.x) |
-                            ((unsigned)__builtin_bit_cast(unsigned short, (_Float16)b.x) << 16);
-        const unsigned wy = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)a.y) |
-                            ((unsigned)__builtin_bit_cast(unsigned short, (_Float16)b.y) << 16);
+        auto h = [](float v) {   // (a NaN stays NaN)
+            const float cl = fminf(fmaxf(v, -65504.0f), 65504.0f);
+            return (unsigned)__builtin_bit_cast(unsigned short, (_Float16)(v == v ? cl : v));
+        };
+        const unsigned wx = h(a.x) | (h(b.x) << 16);
+        const unsigned wy = h(a.y) | (h(b.y) << 16);
         pix[p] = make_float2(__uint_as_float(wx), __uint_as_float(wy));
     } else {
         pix[p] = a;

@@ -649,6 +649,29 @@ def test_backproject_reduced_precision(ctx, dt, tol, name, mode, nv):
     assert r <= tol
 
 
+def test_backproject_fp16_out_of_range_correction(ctx):
+    """One correction pixel far beyond the FP16 range (as a ray that grazes
+    the volume, with a tiny length to divide by, can produce): the
+    packed-half engine's table clamps finite values to the FP16 range, so the
+    16-bit update stays finite everywhere, and voxels whose footprint misses
+    that pixel match the FP32 update as in the plain case."""
+    g = GEOMS["small"]
+    x = torch.as_tensor(phantom(g, seed=18, scale=0.3).numpy()).half()
+    corr = np.random.default_rng(9).standard_normal((1, *g.proj_shape[1:])).astype(np.float32) * 0.01
+    big = corr.copy()
+    big[0, g.nrows // 2, g.ncols // 2] = 1e9
+    v16 = x.cuda()
+    ctx.backproject_voxel(g, v16, dev(big), view0=2, nviews=1, beta=0.7, mode=1)
+    got = host(v16)
+    assert np.isfinite(got).all()
+    x32 = x.float().numpy()
+    ref = orc.backproject_voxel(g, x32, corr, view0=2, nviews=1, beta=0.7, mode=1, prec=32)
+    ref_big = orc.backproject_voxel(g, x32, big, view0=2, nviews=1, beta=0.7, mode=1, prec=32)
+    away = ref_big == ref           # voxels the large pixel does not reach
+    assert away.sum() > 0.5 * away.size and (~away).sum() > 0
+    assert rel(got[away], ref[away]) <= 3e-3
+
+
 @pytest.mark.parametrize("name", ["small", "fine"])
 @pytest.mark.parametrize("x0", [3e-6, 0.01])
 def test_stochastic_rounding_is_unbiased(ctx, name, x0):
