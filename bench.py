@@ -7,17 +7,21 @@ Workload (synthetic, seeded): 512^3 volume of 0.5 mm voxels holding the
 1024x1024 detector of 0.4 mm pixels, SID 1000 mm / SDD 1500 mm; Poisson noise
 with I0 = 1e5 photons per ray, then the log transform.  One step = one outer
 iteration: a SART sweep over all 1024 views in the reference's view-by-view
-order (relaxation 2/32, the reference's 2/Lip_con), TV-FGP with 20 inner
-iterations (lambda 1e-3) and the FISTA momentum, all device resident.
-Inputs are in HBM before timing starts.
+order (relaxation 2/Lip with Lip from 30 power iterations of the operator one
+sweep applies, in place of the reference's constant Lip_con = 32), TV-FGP with
+20 inner iterations (lambda 1e-3) and the FISTA momentum, all device
+resident.  Inputs are in HBM before timing starts.
 
 value = fwd+back GRays/s of the FP32 path = rays per sweep / (time of the
 Siddon correction kernel + time of the voxel-driven update kernel), both timed
 live with HIP events on the context stream.  The same run reports the FP16
-path, seconds per iteration of both, the roofline of the dominant kernel,
-FDK (Hann) over the same projections, the autograd pair (ray sums + matched
-adjoint) on a sample of views, and the CPU oracle (OpenMP restatement of the
-reference kernels) on a bounded sample of the same sweep.
+and BF16 paths, seconds per iteration of each, the roofline of the dominant
+kernel and of both sweep kernels in FP32 and FP16, the error of three
+iterations against the FP64 path, FDK (Hann) over the same projections, the
+autograd pair (ray sums + matched adjoint) and the multi-source pair on
+samples of views, other beam styles, file I/O, the 128^3 / 180-view SART
+workload, and the CPU oracle (OpenMP restatement of the reference kernels) on
+bounded samples of the same workloads.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
