@@ -389,9 +389,9 @@ template <int NP>
 __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
                                           int cbase, float zl, float zh, float nA, const unsigned *lo2,
                                           const unsigned *hi2, float &wk, float &lk) {
-    for (int j = z0; j <= z1; ++j) {
-        const float2 rt = sr[j];
-        const uint2 *prow = sp + j * LP + cbase;
+    const uint2 *prow = sp + (uint32_t)(z0 * LP + cbase);
+    for (const float2 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
+        const float2 rt = *rp;
         uint2 pv[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) pv[p] = prow[2 * p];
@@ -428,9 +428,9 @@ template <int NC>
 __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
                                         int cbase, float zl, float zh, const float *lo, const float *hi, float &wk,
                                         float &lk) {
-    for (int j = z0; j <= z1; ++j) {
-        const float2 rt = sr[j];
-        const float2 *prow = sp + j * LP + cbase;
+    const float2 *prow = sp + (uint32_t)(z0 * LP + cbase);
+    for (const float2 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
+        const float2 rt = *rp;
         float2 pv[NC];
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) pv[cc] = prow[cc];
@@ -560,6 +560,24 @@ __device__ __forceinline__ uint32_t vox_index(const DGeo<float> &d, int ix, int 
     else if constexpr (ZQ) return ((((uint32_t)iz >> 2) * N + (uint32_t)iy) * M + (uint32_t)ix) * 4u + ((uint32_t)iz & 3u);
     else return ((uint32_t)iz * N + (uint32_t)iy) * M + (uint32_t)ix;
 }
+// vox_index split into the part set by the z index (uniform over a
+// workgroup's z-run step) and the part set by (x, y) (per lane):
+// vox_index = vox_plane(iz) + vox_lane(ix, iy).  The voxel's address is then
+// a scalar base plus a 32-bit lane offset, with no per-voxel vector address
+// arithmetic.
+template <bool ZQ>
+__device__ __forceinline__ uint32_t vox_plane(const DGeo<float> &d, int iz) {
+    const uint32_t MN = (uint32_t)d.M * (uint32_t)d.N;
+    if constexpr (ZQ) return ((uint32_t)iz >> 2) * MN * 4u + ((uint32_t)iz & 3u);
+    else return (uint32_t)iz * MN;
+}
+template <bool ZQ, bool TR = false>
+__device__ __forceinline__ uint32_t vox_lane(const DGeo<float> &d, int ix, int iy) {
+    const uint32_t M = d.M, N = d.N;
+    if constexpr (ZQ && TR) return ((uint32_t)ix * N + (uint32_t)iy) * 4u;
+    else if constexpr (ZQ) return ((uint32_t)iy * M + (uint32_t)ix) * 4u;
+    else return (uint32_t)iy * M + (uint32_t)ix;
+}
 
 template <typename SV, bool ONE, bool H, bool ZQ, bool TR = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void k_voxel_bp_lds(DGeo<float> d, const float *__restrict__ ang,
@@ -613,6 +631,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
 #pragma unroll
     for (int k = 0; k < (ONE ? 1 : ZR); ++k) { W[k] = 0.0f; L[k] = 0.0f; }
     const int64_t P = (int64_t)d.R * d.ZP;
+    const uint32_t lane_b = vox_lane<ZQ, TR>(d, ix, iy) * (uint32_t)sizeof(SV);
     for (int q = 0; q < nv; ++q) {
         const int v = first + q * stride;
         const float s = ang[2 * v], c = ang[2 * v + 1];
@@ -727,31 +746,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             // in the corner ratio r, so the extremes over the eight corners
             // come from the extreme ratios: min(w(rmin), w(rmax)) is the lowest
             // shadow of a lower face, the max the highest of an upper face.
-            const float zrow = -d.zmin * d.idv;
+            // Shadow rows: shadow_lo / shadow_hi with their 0.01-row margins
+            // folded into the offset.  The row range only has to cover the
+            // rows whose rays cross the voxel (any other row adds exactly
+            // zero), which the margins keep through the different rounding.
+            const float zrow_lo = -d.zmin * d.idv - 0.01f, zrow_hi = -d.zmin * d.idv + 0.01f;
+            float bz = d.bz;
+            asm volatile("" : "+v"(bz));   // a lane register: one scalar operand per FMA
             float zk = (float)iz0;
-            float zh = fmaf(zk, d.vz, d.bz);
+            float zh = fmaf(zk, d.vz, bz);
             int z0n;   // shadow of the next voxel's lower face
             {
                 const float e = zh - szs;
-                z0n = shadow_lo(fmaf(hmin(fmaf(rmin, e, zh), fmaf(rmax, e, zh)), d.idv, zrow));
+                z0n = (int)ceilf(fmaf(hmin(fmaf(rmin, e, zh), fmaf(rmax, e, zh)), d.idv, zrow_lo));
             }
 #pragma unroll 1
             for (int k = 0; k < nz; ++k) {
                 const float zl = zh;
                 zk += 1.0f;
-                zh = fmaf(zk, d.vz, d.bz);
+                zh = fmaf(zk, d.vz, bz);
                 const float e = zh - szs;
                 const float wa = fmaf(rmin, e, zh), wb = fmaf(rmax, e, zh);
                 int z0 = z0n;
-                int z1 = shadow_hi(fmaf(hmax(wa, wb), d.idv, zrow));
-                z0n = shadow_lo(fmaf(hmin(wa, wb), d.idv, zrow));
+                int z1 = (int)floorf(fmaf(hmax(wa, wb), d.idv, zrow_hi));
+                z0n = (int)ceilf(fmaf(hmin(wa, wb), d.idv, zrow_lo));
                 // single image: the voxel's old value is loaded here, its
                 // latency hidden behind the chord sums (mode 0 leaves it
                 // unset and unused: an initialising move would wait for the
                 // previous voxel's load)
+                // the voxel: scalar plane base plus the lane's byte offset
+                SV *const fv = reinterpret_cast<SV *>(reinterpret_cast<char *>(f + vox_plane<ZQ>(d, iz0 + k)) + lane_b);
                 typename RawOf<SV>::type xr;
                 if constexpr (ONE) {
-                    if (mode != 0) xr = ld_raw(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k));
+                    if (mode != 0) xr = ld_raw(fv, 0);
                 }
                 if (d.tb) {
                     const float cz = d.bz + d.vz * (0.5f + (float)(iz0 + k));
@@ -774,7 +801,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                     else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
                 }
-                if constexpr (ONE) bp_update_x(f, vox_index<ZQ, TR>(d, ix, iy, iz0 + k), xr, wk, lk, beta, mode, th, wsc);
+                if constexpr (ONE) bp_update_x(fv, 0, xr, wk, lk, beta, mode, th, wsc);
                 else { W[k] = wk; L[k] = lk; }
             }
         }
@@ -1008,6 +1035,9 @@ static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *
 
 static bool lds_eligible(const ct_context *ctx, const DGeo<float> &d, int nimg) {
     if (d.style != CT_CONE) return false; // the footprint tables are the cone-beam projection
+    // the per-lane voxel offset (vox_lane, in bytes) must fit 32 bits: one
+    // z-quad slab of up to 4-byte voxels
+    if ((int64_t)d.M * d.N * 4 * 4 > (int64_t)UINT32_MAX) return false;
     return ctx->math == CT_MATH_FAST && nimg <= 65535 && bp_lds_fits(d, nimg == 1 ? zrun<true>() : zrun<false>());
 }
 
