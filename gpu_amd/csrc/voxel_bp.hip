@@ -272,8 +272,18 @@ __device__ __forceinline__ void voxel_wl_fast(const DGeo<float> &d, float cx, fl
 
 // Per-view tables of the LDS-staged fast backprojector, built in one pass
 // over the subset's correction images: per pixel (c*|S-P|, |S-P|), per column
-// (qx, qy, 1/ex, 1/ey), per row and source (qz, 1/ez).  The pixel, slab-plane
-// and source coordinates use the reference's arithmetic (see voxel_wl_fast).
+// (qx, qy, 1/ex, 1/ey), per row and source (qa, qb, 1/ez).  The pixel,
+// slab-plane and source coordinates use the reference's arithmetic (see
+// voxel_wl_fast).
+// The row entry orders the ray's two z-plane crossings of a voxel with lower
+// face zl: its alpha interval is [(zl - qa)/ez, (zl - qb)/ez], with
+// (qa, qb) = (qz, qz - vz) for a rising ray (1/ez > 0) and (qz - vz, qz) for
+// a falling one, so the update needs no min/max per row.  A horizontal ray
+// (1/ez = +-inf) crosses a voxel exactly when zl < qz < zl + vz; at a tie the
+// reference's min/max of (+-inf, NaN) leaves it out.  The infinite products
+// then give +-inf, and NaN (taken by min/max as "no bound") exactly where zl
+// equals qa or qb, so qa and qb are moved one float outwards (qz down, qz - vz
+// up): every tie then lands on the empty side.
 // PK (the packed-half engine of reduced-precision volumes): the pixel entry
 // holds the column pair (i, i + 1) as half pairs, (c*|S-P|, c*|S-P|) and
 // (|S-P|, |S-P|) rounded to nearest (zero past the last column), so that the
@@ -285,7 +295,7 @@ template <bool PK>
 __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__restrict__ ang,
                                                  const float *__restrict__ corr, int first, int stride,
                                                  float2 *__restrict__ pix, float4 *__restrict__ colT,
-                                                 float2 *__restrict__ rowT) {
+                                                 float4 *__restrict__ rowT) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, img = blockIdx.z;
     if (i >= d.R) return;
@@ -325,7 +335,18 @@ __global__ __launch_bounds__(256) void k_bp_prep(DGeo<float> d, const float *__r
         pix[p] = a;
     }
     if (j == 0 && src == 0) colT[(int64_t)q * d.R + i] = ct;
-    if (i == 0) rowT[(int64_t)img * d.ZP + j] = make_float2(qz, 1.0f / ez);
+    if (i == 0) {
+        const float kz = 1.0f / ez, qd = qz - d.vz;
+        float qa = kz > 0.0f ? qz : qd, qb = kz > 0.0f ? qd : qz;
+        if (kz == __builtin_inff()) {
+            qa = nextafterf(qz, -__builtin_inff());
+            qb = nextafterf(qd, __builtin_inff());
+        } else if (kz == -__builtin_inff()) {
+            qa = nextafterf(qd, __builtin_inff());
+            qb = nextafterf(qz, -__builtin_inff());
+        }
+        rowT[(int64_t)img * d.ZP + j] = make_float4(qa, qb, kz, 0.0f);
+    }
 }
 
 constexpr int BPX = 16, BPY = 16;
@@ -351,25 +372,15 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned pk_rtz(float a, float b) {
     return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
 }
-__device__ __forceinline__ unsigned pk_min(unsigned a, unsigned b) {
+// min(a, b.hi) and max(a, b.lo) per half (one half of b broadcast)
+__device__ __forceinline__ unsigned pk_min_h(unsigned a, unsigned b) {
     unsigned r;
-    asm("v_pk_min_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    asm("v_pk_min_f16 %0, %1, %2 op_sel:[0,1] op_sel_hi:[1,1]" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
-__device__ __forceinline__ unsigned pk_max(unsigned a, unsigned b) {
+__device__ __forceinline__ unsigned pk_max_l(unsigned a, unsigned b) {
     unsigned r;
-    asm("v_pk_max_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-// min / max of the two halves of a, in both halves
-__device__ __forceinline__ unsigned pk_min_bc(unsigned a) {
-    unsigned r;
-    asm("v_pk_min_f16 %0, %1, %1 op_sel:[0,1] op_sel_hi:[1,0]" : "=v"(r) : "v"(a));
-    return r;
-}
-__device__ __forceinline__ unsigned pk_max_bc(unsigned a) {
-    unsigned r;
-    asm("v_pk_max_f16 %0, %1, %1 op_sel:[0,1] op_sel_hi:[1,0]" : "=v"(r) : "v"(a));
+    asm("v_pk_max_f16 %0, %1, %2 op_sel:[0,0] op_sel_hi:[1,0]" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
 // max(a - b, 0) per half, for a - b < 1 (output clamp to [0, 1]); local
@@ -383,25 +394,24 @@ __device__ __forceinline__ float dot2(unsigned a, unsigned b, float c) {
     return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c, false);
 }
 
-// rows of the packed-half engine: per row (qz, kl/ez) from LDS, per column
-// pair the staged (c*|S-P|, |S-P|) half pairs
+// rows of the packed-half engine: per row (qa, qb, kl/ez) from LDS, per
+// column pair the staged (c*|S-P|, |S-P|) half pairs
 template <int NP>
-__device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
-                                          int cbase, float zl, float zh, float nA, const unsigned *lo2,
+__device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const float4 *__restrict__ sr, int z0, int z1,
+                                          int cbase, float zl, float nA, const unsigned *lo2,
                                           const unsigned *hi2, float &wk, float &lk) {
     const uint2 *prow = sp + (uint32_t)(z0 * LP + cbase);
-    for (const float2 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
-        const float2 rt = *rp;
+    for (const float4 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
+        const float4 rt = *rp;
         uint2 pv[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) pv[p] = prow[2 * p];
-        // local crossings (z - qz)*(kl/ez) + nA; a horizontal ray (1/ez
-        // infinite) gives +-inf, or NaN exactly on a plane, as in FP32
-        const unsigned az = pk_rtz(fmaf(zl - rt.x, rt.y, nA), fmaf(zh - rt.x, rt.y, nA));
-        const unsigned loz = pk_min_bc(az), hiz = pk_max_bc(az);
+        // the voxel's local alpha interval on this row's ray, (lower, upper)
+        // in (lo, hi) half (a horizontal ray: +-inf, or NaN for no bound)
+        const unsigned az = pk_rtz(fmaf(zl - rt.x, rt.z, nA), fmaf(zl - rt.y, rt.z, nA));
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            const unsigned span = pk_span(pk_min(hi2[p], hiz), pk_max(lo2[p], loz));
+            const unsigned span = pk_span(pk_min_h(hi2[p], az), pk_max_l(lo2[p], az));
             wk = dot2(span, pv[p].x, wk);
             lk = dot2(span, pv[p].y, lk);
         }
@@ -425,21 +435,19 @@ __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const fl
 // per-voxel table path for that image.
 
 template <int NC>
-__device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float2 *__restrict__ sr, int z0, int z1,
-                                        int cbase, float zl, float zh, const float *lo, const float *hi, float &wk,
+__device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float4 *__restrict__ sr, int z0, int z1,
+                                        int cbase, float zl, const float *lo, const float *hi, float &wk,
                                         float &lk) {
     const float2 *prow = sp + (uint32_t)(z0 * LP + cbase);
-    for (const float2 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
-        const float2 rt = *rp;
+    for (const float4 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
+        const float4 rt = *rp;
         float2 pv[NC];
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) pv[cc] = prow[cc];
-        // the two z-plane crossings of the row's ray, in the reference's
-        // product form: a horizontal ray (1/ez infinite) gives +-inf, or NaN
-        // exactly on a plane, and min/max then resolve the slab as the
-        // reference does
-        const float az0 = (zl - rt.x) * rt.y, az1 = (zh - rt.x) * rt.y;
-        const float loz = hmin(az0, az1), hiz = hmax(az0, az1);
+        // the voxel's alpha interval on this row's ray, from the ordered
+        // plane offsets in the reference's product form (a horizontal ray:
+        // +-inf, or NaN, which min/max take as no bound)
+        const float loz = (zl - rt.x) * rt.z, hiz = (zl - rt.y) * rt.z;
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
             // alpha spans are below 1 (a chord is a fraction of |S - P|)
@@ -583,11 +591,11 @@ template <typename SV, bool ONE, bool H, bool ZQ, bool TR = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void k_voxel_bp_lds(DGeo<float> d, const float *__restrict__ ang,
                                                       SV *__restrict__ f, const float2 *__restrict__ pixT,
                                                       const float4 *__restrict__ colT,
-                                                      const float2 *__restrict__ rowT, int first, int stride,
+                                                      const float4 *__restrict__ rowT, int first, int stride,
                                                       int nv, float beta, int mode) {
     __shared__ float2 sp[LH * LP];   // (c*|S-P|, |S-P|); H: column-pair halves
     __shared__ float4 sc[LP];
-    __shared__ float2 sr[LH];        // (qz, 1/ez); H: (qz, kl/ez)
+    __shared__ float4 sr[LH];        // (qa, qb, 1/ez); H: (qa, qb, kl/ez)
     uint2 *const sph = reinterpret_cast<uint2 *>(sp);
     const int tid = threadIdx.y * BPX + threadIdx.x;
     const int3 tile = xcd_tile();   // each XCD: a contiguous z range of tiles
@@ -683,7 +691,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             const int nrow = R1 - R0 + 1;
             if (ncw <= 0 || nrow <= 0) continue;   // footprint off the detector (uniform)
             const float2 *pimg = pixT + (int64_t)img * P;
-            const float2 *rimg = rowT + (int64_t)img * d.ZP;
+            const float4 *rimg = rowT + (int64_t)img * d.ZP;
             const float4 *cimg = colT + (int64_t)q * d.R;
             // TB source culling, TB/voxel/kernel_IterativeRecon_TBCT_voxel.cu:618-627
             float rk = 0.0f;
@@ -702,8 +710,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             }
             if (tid < LP) sc[tid] = tid < ncw ? cimg[C0 + tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (tid < nrow) {
-                const float2 rt = rimg[R0 + tid];
-                sr[tid] = H ? make_float2(rt.x, kl * rt.y) : rt;
+                const float4 rt = rimg[R0 + tid];
+                sr[tid] = H ? make_float4(rt.x, rt.y, kl * rt.z, 0.0f) : rt;
             }
             __syncthreads();
             if (!active || yoff) continue;
@@ -786,20 +794,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                     const float lower = (cz - d.vz - d.zmax) * rk + d.zmax;
                     if (szs < lower || szs > upper) continue;
                 }
-                if ((z0 < 0 && z1 < 0) || (z0 > d.ZP - 1 && z1 > d.ZP - 1)) continue;
+                // rows of the shadow inside the staged window; a shadow off
+                // the detector leaves the range empty (no rows, L = 0, below
+                // the update threshold)
                 z0 = max(z0, R0) - R0;
                 z1 = min(z1, R1) - R0;
                 float wk = ONE ? 0.0f : W[k], lk = ONE ? 0.0f : L[k];
                 if constexpr (H) {
-                    if (NC == 8) bp_rows_h<4>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
-                    else if (NC >= 5) bp_rows_h<3>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
-                    else bp_rows_h<2>(sph, sr, z0, z1, cbase, zl, zh, nA, lo2, hi2, wk, lk);
+                    if (NC == 8) bp_rows_h<4>(sph, sr, z0, z1, cbase, zl, nA, lo2, hi2, wk, lk);
+                    else if (NC >= 5) bp_rows_h<3>(sph, sr, z0, z1, cbase, zl, nA, lo2, hi2, wk, lk);
+                    else bp_rows_h<2>(sph, sr, z0, z1, cbase, zl, nA, lo2, hi2, wk, lk);
                 } else {
-                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
-                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, zh, lo, hi, wk, lk);
+                    if (NC == 8) bp_rows<8>(sp, sr, z0, z1, cbase, zl, lo, hi, wk, lk);
+                    else if (NC == 6) bp_rows<6>(sp, sr, z0, z1, cbase, zl, lo, hi, wk, lk);
+                    else if (NC == 5) bp_rows<5>(sp, sr, z0, z1, cbase, zl, lo, hi, wk, lk);
+                    else if (NC == 4) bp_rows<4>(sp, sr, z0, z1, cbase, zl, lo, hi, wk, lk);
+                    else bp_rows<3>(sp, sr, z0, z1, cbase, zl, lo, hi, wk, lk);
                 }
                 if constexpr (ONE) bp_update_x(fv, 0, xr, wk, lk, beta, mode, th, wsc);
                 else { W[k] = wk; L[k] = lk; }
@@ -1009,12 +1019,12 @@ static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *
     const size_t P = (size_t)d.R * d.ZP;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t pix_b = up(sizeof(float2) * P * nimg), col_b = up(sizeof(float4) * (size_t)d.R * nv);
-    const size_t row_b = up(sizeof(float2) * (size_t)d.ZP * nimg);
+    const size_t row_b = up(sizeof(float4) * (size_t)d.ZP * nimg);
     char *tab = (char *)slot(ctx, SLOT_BPTAB, pix_b + col_b + row_b, &st);
     if (!tab) return st;
     float2 *pix = (float2 *)tab;
     float4 *colT = (float4 *)(tab + pix_b);
-    float2 *rowT = (float2 *)(tab + pix_b + col_b);
+    float4 *rowT = (float4 *)(tab + pix_b + col_b);
     ProfScope ps(ctx, CT_K_VOXEL_BP);
     // reduced-precision storage takes the packed-half chord engine
     constexpr bool H = !std::is_same<SV, float>::value;
@@ -1024,11 +1034,11 @@ static ct_status run_bp_lds(ct_context *ctx, const DGeo<float> &d, const float *
     dim3 lgrid((d.M + BPX - 1) / BPX, (d.N + BPY - 1) / BPY, (d.Z + zr - 1) / zr);
     if (nimg == 1)
         hipLaunchKernelGGL((k_voxel_bp_lds<SV, true, H, ZQ, TR>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
-                           (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
+                           (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float4 *)rowT, first, stride,
                            nv, (float)beta, mode);
     else
         hipLaunchKernelGGL((k_voxel_bp_lds<SV, false, H, ZQ, TR>), lgrid, dim3(BPX, BPY, 1), 0, ctx->stream, d, ang,
-                           (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float2 *)rowT, first, stride,
+                           (SV *)vol, (const float2 *)pix, (const float4 *)colT, (const float4 *)rowT, first, stride,
                            nv, (float)beta, mode);
     return launch_check(ctx, "voxel backprojection");
 }
