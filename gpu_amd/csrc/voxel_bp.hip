@@ -400,7 +400,7 @@ template <int NP>
 __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const float4 *__restrict__ sr, int z0, int z1,
                                           int cbase, float zl, float nA, const unsigned *lo2,
                                           const unsigned *hi2, float &wk, float &lk) {
-    const uint2 *prow = sp + (uint32_t)(z0 * LP + cbase);
+    const uint2 *prow = sp + (__umul24((uint32_t)z0, LP) + cbase);
     for (const float4 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
         const float4 rt = *rp;
         uint2 pv[NP];
@@ -438,7 +438,7 @@ template <int NC>
 __device__ __forceinline__ void bp_rows(const float2 *__restrict__ sp, const float4 *__restrict__ sr, int z0, int z1,
                                         int cbase, float zl, const float *lo, const float *hi, float &wk,
                                         float &lk) {
-    const float2 *prow = sp + (uint32_t)(z0 * LP + cbase);
+    const float2 *prow = sp + (__umul24((uint32_t)z0, LP) + cbase);
     for (const float4 *rp = sr + z0, *const re = sr + z1; rp <= re; ++rp, prow += LP) {
         const float4 rt = *rp;
         float2 pv[NC];
@@ -754,11 +754,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
             // in the corner ratio r, so the extremes over the eight corners
             // come from the extreme ratios: min(w(rmin), w(rmax)) is the lowest
             // shadow of a lower face, the max the highest of an upper face.
-            // Shadow rows: shadow_lo / shadow_hi with their 0.01-row margins
-            // folded into the offset.  The row range only has to cover the
-            // rows whose rays cross the voxel (any other row adds exactly
-            // zero), which the margins keep through the different rounding.
-            const float zrow_lo = -d.zmin * d.idv - 0.01f, zrow_hi = -d.zmin * d.idv + 0.01f;
+            // Shadow rows, counted from the staged window's first row R0:
+            // shadow_lo / shadow_hi with their 0.01-row margins folded into
+            // the offset.  The row range only has to cover the rows whose
+            // rays cross the voxel (any other row adds exactly zero), which
+            // the margins keep through the different rounding.
+            const float zrow = -d.zmin * d.idv - (float)R0;
+            const float zrow_lo = zrow - 0.01f, zrow_hi = zrow + 0.01f;
             float bz = d.bz;
             asm volatile("" : "+v"(bz));   // a lane register: one scalar operand per FMA
             float zk = (float)iz0;
@@ -797,8 +799,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4
                 // rows of the shadow inside the staged window; a shadow off
                 // the detector leaves the range empty (no rows, L = 0, below
                 // the update threshold)
-                z0 = max(z0, R0) - R0;
-                z1 = min(z1, R1) - R0;
+                z0 = max(z0, 0);
+                z1 = min(z1, nrow - 1);
                 float wk = ONE ? 0.0f : W[k], lk = ONE ? 0.0f : L[k];
                 if constexpr (H) {
                     if (NC == 8) bp_rows_h<4>(sph, sr, z0, z1, cbase, zl, nA, lo2, hi2, wk, lk);
