@@ -423,8 +423,9 @@ __device__ __forceinline__ void bp_rows_h(const uint2 *__restrict__ sp, const fl
 // workgroup stages its detector footprint - the projection of the tile's box,
 // widened by one pixel each way - from the per-view tables into LDS: pixel
 // values (c*|S-P|, |S-P|) in rows of LP entries (columns past the footprint
-// are zero), the column table (qx, qy, 1/ex, 1/ey) and the row table
-// (qz, 1/ez).  Each voxel's bounding box lies inside the footprint, so every
+// are zero), the column table (qx, qy, 1/ex, 1/ey) and the ordered row table
+// (qa, qb, 1/ez) of k_bp_prep.  Each voxel's bounding box lies inside the
+// footprint, so every
 // pixel read of the chord sum is an unconditional LDS read.  The per-lane
 // column count is raised to a wave-uniform NC in {4, 5, 6, 8}; columns past a
 // lane's own bounding box get an empty alpha interval and add exactly zero.
