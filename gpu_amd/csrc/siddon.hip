@@ -672,28 +672,6 @@ static inline size_t col_lds_bytes(int M, int N) {
     return sizeof(float) * 2 * COL_B * COL_W + col_dyn_bytes(M, N);
 }
 
-template <typename SV> __device__ __forceinline__ float4 load_quad(__amdgpu_buffer_rsrc_t r, uint32_t off);
-template <> __device__ __forceinline__ float4 load_quad<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
-}
-template <> __device__ __forceinline__ float4 load_quad<__half>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    // the halves are taken out as integers (a bit_cast of the vector's
-    // elements to __half2 loses the second element)
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    const unsigned x = v[0], y = v[1];
-    return make_float4(__half2float(__ushort_as_half((unsigned short)(x & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short)(x >> 16))),
-                       __half2float(__ushort_as_half((unsigned short)(y & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short)(y >> 16))));
-}
-template <> __device__ __forceinline__ float4 load_quad<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    const unsigned x = v[0], y = v[1];
-    return make_float4(__uint_as_float(x << 16), __uint_as_float(x & 0xffff0000u), __uint_as_float(y << 16),
-                       __uint_as_float(y & 0xffff0000u));
-}
-
 // Element type of the staged windows: FP16 volumes stage their halves as
 // loaded (the walk's FMAs take FP16 operands directly), the others FP32.
 template <typename SV> struct StageOf { using type = float; };
@@ -869,7 +847,10 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     }
     // a quad outside the volume reads as zeros: its index is clamped to -1 or
     // ZQ4, whose offsets lie past the buffer's end (the volume is under 2 GiB)
-    using SR = typename std::conditional<SB == 2, uint2, float4>::type;
+    // Staged quads stay in registers as loaded (FP32 quads, or the 16-bit
+    // volume's raw halves; BF16 is widened only at the LDS write), so no
+    // instruction waits for them before the commit.
+    using SR = typename std::conditional<ES == 2, uint2, float4>::type;
     SR sreg[SP];
     auto issue = [&](int b) {
 #pragma unroll
@@ -878,11 +859,13 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
                 const int e = b * CB + sk[p];
                 const int qz = min(max((lw[e] >> QSH) + sq[p], -1), ZQ4);
                 const uint32_t off = ls[e] + (unsigned)qz * QS;
-                if constexpr (SB == 2) {
+                if constexpr (ES == 2) {
                     const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
                     sreg[p] = make_uint2(v[0], v[1]);
                 } else {
-                    sreg[p] = load_quad<SV>(rs, off);
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+                    sreg[p] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                                          __uint_as_float(v[3]));
                 }
             }
         }
@@ -890,8 +873,17 @@ __global__ __launch_bounds__(COL_ROWS) void k_project_col(DGeo<float> d, const f
     auto commit = [&](auto bufc) {
         constexpr int buf = decltype(bufc)::value;
 #pragma unroll
-        for (int p = 0; p < SP; ++p)
-            if (sk[p] >= 0) *reinterpret_cast<SR *>(stg + (buf * CB * COL_W + so[p])) = sreg[p];
+        for (int p = 0; p < SP; ++p) {
+            if (sk[p] < 0) continue;
+            SE *const dst = stg + (buf * CB * COL_W + so[p]);
+            if constexpr (std::is_same<SV, __hip_bfloat16>::value) {
+                const uint2 r = sreg[p];
+                *reinterpret_cast<float4 *>(dst) = make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xffff0000u),
+                                                               __uint_as_float(r.y << 16), __uint_as_float(r.y & 0xffff0000u));
+            } else {
+                *reinterpret_cast<SR *>(dst) = sreg[p];
+            }
+        }
     };
     __syncthreads();   // the list is complete
     const int nb = KP / CB;
