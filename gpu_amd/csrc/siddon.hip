@@ -651,8 +651,8 @@ constexpr int COL_W = 4 * COL_NQ + 4;
 // List segments per staged batch: 16 (one barrier per 16 segments, and a
 // staging load's eight-lane groups cover 16 segments), or 8 for BF16
 // volumes, whose FP32 windows at 16 would leave four workgroups per CU
-// instead of six (BF16 correction per view at 512^3 / 1024^2: 0.327 ms at 8,
-// 0.359 at 16; FP32 0.382 -> 0.342 ms and FP16 0.286 -> 0.261 ms from 8 to 16).
+// instead of six (correction per view at 512^3 / 1024^2: FP32 0.382 -> 0.342
+// ms and FP16 0.286 -> 0.261 ms from 8 to 16; BF16 0.263 ms at either).
 template <typename SV> constexpr int col_batch() { return std::is_same<SV, __hip_bfloat16>::value ? 8 : 16; }
 // staged quads per thread and batch
 template <typename SV> constexpr int col_sp() { return (col_batch<SV>() * COL_NQ + COL_ROWS - 1) / COL_ROWS; }
