@@ -672,6 +672,39 @@ def test_backproject_fp16_out_of_range_correction(ctx):
     assert rel(got[away], ref[away]) <= 3e-3
 
 
+@pytest.mark.parametrize("dt,tol", [(torch.float32, 1e-5), (torch.float16, 3e-3)])
+@pytest.mark.parametrize("nz,tie", [(16, True), (15, False)])
+def test_backproject_horizontal_row(ctx, dt, tol, nz, tie):
+    """A detector row level with the source (21 rows, source at z = 0) casts
+    horizontal rays, whose 1/ez is infinite.  With 16 slices those rays lie
+    exactly on the face between slices 7 and 8, where the reference's min/max
+    of (+-inf, NaN) gives neither voxel a chord: a correction on that row
+    alone changes nothing (the oracle agrees).  With 15 slices they run
+    through the middle of slice 7, whose voxels take full chords.  The
+    LDS-staged kernels (FP32 and the FP16 packed-half engine) resolve both
+    from their ordered row table: at the tie the volume comes back
+    bit-identical (a chord given to either voxel would move it by ~0.007),
+    otherwise only slice 7 moves and the volume matches the oracle's to the
+    storage type's bar."""
+    g = small_geom(nz=nz, nrows=21)
+    xs = torch.as_tensor(phantom(g, seed=14, scale=0.3).numpy()).to(dt)
+    x_in = xs.float().numpy()
+    corr = np.zeros((1, *g.proj_shape[1:]), np.float32)
+    corr[0, g.nrows // 2, :] = 0.01
+    ref = orc.backproject_voxel(g, x_in, corr, view0=2, nviews=1, beta=0.7, mode=1, prec=32)
+    v = xs.cuda()
+    ctx.backproject_voxel(g, v, dev(corr), view0=2, nviews=1, beta=0.7, mode=1)
+    got = host(v)
+    if tie:
+        assert np.array_equal(ref, x_in)
+        assert np.array_equal(got, x_in)
+    else:
+        moved = ref != x_in
+        assert moved.sum() > 100 and not moved[:7].any() and not moved[8:].any()
+        assert np.array_equal(got[:7], x_in[:7]) and np.array_equal(got[8:], x_in[8:])
+        assert rel(got, ref) <= tol
+
+
 @pytest.mark.parametrize("name", ["small", "fine"])
 @pytest.mark.parametrize("x0", [3e-6, 0.01])
 def test_stochastic_rounding_is_unbiased(ctx, name, x0):
