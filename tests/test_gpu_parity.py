@@ -705,6 +705,42 @@ def test_backproject_horizontal_row(ctx, dt, tol, nz, tie):
         assert rel(got, ref) <= tol
 
 
+@pytest.mark.parametrize("path", ["exact", "per_voxel"])
+@pytest.mark.parametrize("nz,tie", [(16, True), (15, False)])
+def test_backproject_horizontal_row_other_paths(ctx, path, nz, tie):
+    """The horizontal-row case of test_backproject_horizontal_row on the
+    other two voxel backprojectors (FP32): exact mode, bit-identical to the
+    oracle, and the per-voxel fast kernel, which takes footprints too wide
+    for the LDS window (detector pixels of 0.15 voxel; 151 rows, the middle
+    one level with the source).  At the tie the volume is unchanged;
+    otherwise only slice 7 moves."""
+    if path == "exact":
+        g = small_geom(nz=nz, nrows=21)
+    else:
+        g = small_geom(nz=nz, du=0.15, dv=0.15, ncols=200, nrows=151, nviews=5)
+    x = phantom(g, seed=14, scale=0.3).numpy()
+    corr = np.zeros((1, *g.proj_shape[1:]), np.float32)
+    corr[0, g.nrows // 2, :] = 0.01
+    ref = orc.backproject_voxel(g, x, corr, view0=2, nviews=1, beta=0.7, mode=1, prec=32)
+    v = dev(x)
+    ctx.set_math("exact" if path == "exact" else "fast")
+    try:
+        ctx.backproject_voxel(g, v, dev(corr), view0=2, nviews=1, beta=0.7, mode=1)
+        got = host(v)
+    finally:
+        ctx.set_math("fast")
+    if tie:
+        assert np.array_equal(ref, x) and np.array_equal(got, x)
+    else:
+        moved = ref != x
+        assert moved.sum() > 100 and not moved[:7].any() and not moved[8:].any()
+        assert np.array_equal(got[:7], x[:7]) and np.array_equal(got[8:], x[8:])
+        if path == "exact":
+            assert np.array_equal(got, ref)
+        else:
+            assert rel(got, ref) <= 1e-5
+
+
 @pytest.mark.parametrize("name", ["small", "fine"])
 @pytest.mark.parametrize("x0", [3e-6, 0.01])
 def test_stochastic_rounding_is_unbiased(ctx, name, x0):
